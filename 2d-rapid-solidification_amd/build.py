@@ -1,0 +1,85 @@
+"""Build recipe: compiles the HIP library (gfx950) and the CPU oracle.
+
+    python 2d-rapid-solidification_amd/build.py [--jobs N]
+
+Outputs (git-ignored, shipped with the tree):
+    2d-rapid-solidification_amd/rsolid/librsolid.so   HIP kernels + C ABI
+    oracle/liboracle.so                                CPU FP64 restatement
+Object files go to _build/.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(ROOT, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# packed FP32 instructions are disabled for every HIP translation unit
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+# per-file extras: the FP64 stepper rounds every operation as written (no
+# FMA contraction) so it tracks the FP64 oracle to a few ulp
+PER_FILE = {
+    "rs_kernels.hip": ["-ffp-contract=off"],
+    "rs_params.hip": ["-ffp-contract=off"],
+    "rs_capi.hip": ["-ffp-contract=off"],
+    "rs_tip.hip": ["-ffp-contract=off"],
+}
+SOURCES = ["rs_kernels.hip", "rs_params.hip", "rs_tip.hip", "rs_capi.hip"]
+
+LIB = os.path.join(PKG, "rsolid", "librsolid.so")
+ORACLE_SRC = os.path.join(ROOT, "oracle", "rs_oracle.c")
+ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {os.path.basename(cmd[-1])}")
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+    cmd = [HIPCC, *HIP_FLAGS, *PER_FILE.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build_hip(jobs: int = 4) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB, *objs])
+    return LIB
+
+
+def build_oracle() -> str:
+    # portable x86-64-v3 code: the library is built here and run on the GPU
+    # host's cores as the CPU baseline
+    _run(["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-ffp-contract=off", "-fPIC",
+          "-shared", "-std=c11", "-Wall", "-o", ORACLE_LIB, ORACLE_SRC, "-lm"])
+    return ORACLE_LIB
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--no-oracle", action="store_true")
+    a = ap.parse_args()
+    print(build_hip(a.jobs))
+    if not a.no_oracle:
+        print(build_oracle())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,553 @@
+// rs_capi.hip -- the C ABI (include/rsolid.h): context lifetime, uploads,
+// and the host side of the time loop (fun.cu:431-514 without its output
+// calls). All device work goes to the context's own stream.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <new>
+
+#include "rs_host.hpp"
+
+using namespace rs;
+
+namespace {
+
+int fail(rs_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
+    if (ctx) {
+        char buf[512];
+        if (e != hipSuccess)
+            snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+        else
+            snprintf(buf, sizeof buf, "%s", what);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                              \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return fail((ctx), RS_ERR_HIP, #call, e_); \
+    } while (0)
+
+double tau0_of(const rs_params& p) {
+    const double sw = p.S * p.W0;
+    return (sw * sw) / (p.Gamma * p.muk0);
+}
+
+int check_params(const rs_params& p, std::string* why) {
+    if (p.Nx < 8 || p.Ny < 8) { *why = "grid must be at least 8x8"; return RS_ERR_ARG; }
+    if (p.S < 1 || p.dx <= 0 || p.W0 <= 0) { *why = "S, dx and W0 must be positive"; return RS_ERR_ARG; }
+    if (p.ratio_dxT_dx != 10) { *why = "ratio_dxT_dx must be 10 (10-row latent-heat blocks)"; return RS_ERR_ARG; }
+    if (p.precision != RS_PREC_FP64 && p.precision != RS_PREC_MIXED) { *why = "bad precision"; return RS_ERR_ARG; }
+    if (p.ifTFC && p.Ny < 22) { *why = "coupled T needs Ny >= 22"; return RS_ERR_ARG; }
+    if (p.i_tip_target < 1 || p.i_tip_target > p.Nx - 2) { *why = "i_tip_target outside the grid"; return RS_ERR_ARG; }
+    return RS_OK;
+}
+
+// T window check: the fine window must interpolate from interior coarse
+// columns (the reference reads past the coarse grid otherwise,
+// fun.cu:1588-1599)
+int check_window(rs_ctx* ctx, int ile) {
+    if (!ctx->p.ifTFC) return RS_OK;
+    const int r = ctx->p.ratio_dxT_dx;
+    if (ile - 1 < 0 || (ctx->p.Nx - 2 + ile) / r + 1 > ctx->k.NxT - 1) {
+        char b[256];
+        snprintf(b, sizeof b, "fine window left the temperature sample (i_left_end=%d, NxT=%d)", ile,
+                 ctx->k.NxT);
+        return fail(ctx, RS_ERR_DOMAIN, b);
+    }
+    return RS_OK;
+}
+
+void free_all(rs_ctx* ctx) {
+    double* ds[] = {ctx->psi[0], ctx->psi[1], ctx->c[0], ctx->c[1], ctx->T[0], ctx->T[1],
+                    ctx->lh, ctx->rowd, ctx->outd, ctx->x0, ctx->tfine};
+    for (double* d : ds)
+        if (d) (void)hipFree(d);
+    if (ctx->rowi) (void)hipFree(ctx->rowi);
+    if (ctx->outi) (void)hipFree(ctx->outi);
+    if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
+    for (int q = 0; q < 3; ++q)
+        for (hipEvent_t e : ctx->ev[q]) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+}
+
+int ev_record(rs_ctx* ctx, int which) {
+    std::vector<hipEvent_t>& v = ctx->ev[which];
+    const int need = ctx->nev[which] + 1;
+    if ((int)v.size() < need) {
+        if (v.size() >= 8192) return RS_OK;  // pool full: stop sampling
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreate(&e));
+        v.push_back(e);
+    }
+    HIPCHK(ctx, hipEventRecord(v[ctx->nev[which]], ctx->stream));
+    ctx->nev[which]++;
+    return RS_OK;
+}
+
+StepArgsF64 step_args(rs_ctx* ctx) {
+    const rs_params& p = ctx->p;
+    StepArgsF64 a;
+    a.psi = ctx->psi[ctx->cur];
+    a.c = ctx->c[ctx->cur];
+    a.psi_n = ctx->psi[ctx->cur ^ 1];
+    a.c_n = ctx->c[ctx->cur ^ 1];
+    a.lh = ctx->lh;
+    a.Nx = p.Nx;
+    a.Ny = p.Ny;
+    a.band_h = ctx->band_h;
+    a.nbands = (p.Ny - 2 + a.band_h - 1) / a.band_h;
+    a.lh_last_row = p.ratio_dxT_dx * (ctx->k.NyT - 2);
+    a.lh_on = p.ifTFC;
+    a.ts = make_tempsrc(p, ctx->k, ctx->T[ctx->tcur], ctx->k.i_left_end_interp);
+    a.dt = ctx->k.dt;
+    a.AA = p.AA;
+    a.q_solid = p.q_solid;
+    a.dG_den = ctx->d.dG_den;
+    a.mu_den = ctx->d.mu_den;
+    a.kdc = ctx->d.kdc;
+    a.dx2 = p.dx * p.dx;
+    a.rk = rk_of(p);
+    a.an.ops[0] = ctx->orient[0];
+    a.an.ops[1] = ctx->orient[1];
+    a.an.ops[2] = ctx->orient[3];
+    a.an.ops[3] = ctx->orient[4];
+    a.an.ops[4] = ctx->orient[6];
+    a.an.ops[5] = ctx->orient[7];
+    a.an.ops[6] = a.an.ops[7] = 0.0;
+    a.an.eps1 = p.eps1;
+    a.an.epk1 = p.epk1;
+    a.an.lap_w_axis = p.lap_w_axis;
+    a.an.lap_w_diag = p.lap_w_diag;
+    a.an.inv2dx = 1.0 / (2.0 * p.dx);
+    a.an.dx = p.dx;
+    return a;
+}
+
+int do_pull_back(rs_ctx* ctx, int off) {
+    rs_constants& k = ctx->k;
+    k.i_offset = off;
+    k.i_offset_history += off;
+    if (ctx->p.ifTFC)
+        k.i_left_end = k.i_left_end_initial + k.i_offset_history - (int)(k.l_offset_Vp / ctx->p.dx);
+    const int c0 = ctx->cur;
+    HIPCHK(ctx, launch_pull_back(ctx->psi[c0], ctx->c[c0], ctx->psi[c0 ^ 1], ctx->c[c0 ^ 1],
+                                 ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf, ctx->stream));
+    ctx->cur ^= 1;
+    return check_window(ctx, k.i_left_end);
+}
+
+int do_tip_column(rs_ctx* ctx, int* it) {
+    HIPCHK(ctx, launch_tip_column(ctx->psi[ctx->cur], ctx->p.Nx, ctx->p.Ny, ctx->p.phi_interface,
+                                  ctx->rowi, ctx->outi, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pin, ctx->outi, sizeof(int), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(it, ctx->h_pin, sizeof(int));
+    return RS_OK;
+}
+
+int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
+    const rs_params& p = ctx->p;
+    ctx->k = rs_constants{};
+    rs_derive(&p, &ctx->k);
+    ctx->cur = 0;
+    ctx->tcur = 0;
+    orientation(ctx->orient, 0.0, 0.0, 0.0);
+    HIPCHK(ctx, hipMemcpyAsync(ctx->x0, x0.data(), sizeof(double) * p.Ny, hipMemcpyHostToDevice,
+                               ctx->stream));
+    HIPCHK(ctx, launch_init_fields(ctx->psi[0], ctx->c[0], ctx->x0, p.Nx, p.Ny, p.dx, mode, p.cinf,
+                                   p.c_liq0, p.c_sol0, ctx->stream));
+    if (p.ifTFC) {
+        // NyT_all rows plus the spare row the bilinear read may touch
+        HIPCHK(ctx, launch_init_T(ctx->T[0], ctx->T[1], ctx->k.NxT_all, ctx->k.NyT_all + 1,
+                                  ctx->k.Tcold, p.S, p.W0, p.dx, p.ratio_dxT_dx, p.GG, ctx->stream));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->k.i_left_end_interp = ctx->k.i_left_end;
+    return check_window(ctx, ctx->k.i_left_end);
+}
+
+uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_abi_info(int32_t* s) {
+    if (!s) return RS_ERR_ARG;
+    s[0] = (int32_t)sizeof(rs_params);
+    s[1] = (int32_t)sizeof(rs_constants);
+    s[2] = (int32_t)sizeof(rs_tip_row);
+    s[3] = RS_ABI_VERSION;
+    return RS_OK;
+}
+
+int rs_create(const rs_params* p, rs_ctx** out) {
+    if (!p || !out) return RS_ERR_ARG;
+    *out = nullptr;
+    std::string why;
+    int rc = check_params(*p, &why);
+    if (rc) return rc;
+    rs_ctx* ctx = new (std::nothrow) rs_ctx;
+    if (!ctx) return RS_ERR_NOMEM;
+    ctx->p = *p;
+    rs_derive(p, &ctx->k);
+    ctx->d = derive_constants(*p, ctx->k.dt);
+    orientation(ctx->orient, 0.0, 0.0, 0.0);
+    ctx->n = (int64_t)p->Nx * p->Ny;
+    ctx->nT = (int64_t)ctx->k.NxT_all * ctx->k.NyT_all;
+    if (const char* e = getenv("RS_BAND_H")) {
+        const int v = atoi(e);
+        if (v >= 10 && v % 10 == 0) ctx->band_h = v;
+    }
+    if (const char* e = getenv("RS_WPB")) {
+        const int v = atoi(e);
+        if (v >= 1 && v <= 4) ctx->wpb = v;
+    }
+    hipError_t e;
+    if ((e = hipGetDevice(&ctx->device)) != hipSuccess ||
+        (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+        free_all(ctx);
+        delete ctx;
+        return RS_ERR_HIP;
+    }
+    const size_t fb = sizeof(double) * (size_t)ctx->n;
+    // coarse T: NxT_all x NyT_all plus one spare row (the bilinear read of
+    // row jT+1 reaches row NyT_all, with weight 0, when NyT % 4 == 0)
+    const size_t tb = sizeof(double) * (size_t)(ctx->nT + ctx->k.NxT_all);
+    const size_t lhb = sizeof(double) * (size_t)p->Nx * (size_t)(ctx->k.NyT > 2 ? ctx->k.NyT - 2 : 1);
+    bool ok = hipMalloc(&ctx->psi[0], fb) == hipSuccess && hipMalloc(&ctx->psi[1], fb) == hipSuccess &&
+              hipMalloc(&ctx->c[0], fb) == hipSuccess && hipMalloc(&ctx->c[1], fb) == hipSuccess &&
+              hipMalloc(&ctx->T[0], tb) == hipSuccess && hipMalloc(&ctx->T[1], tb) == hipSuccess &&
+              hipMalloc(&ctx->lh, lhb) == hipSuccess &&
+              hipMalloc(&ctx->rowd, sizeof(double) * p->Ny) == hipSuccess &&
+              hipMalloc(&ctx->rowi, sizeof(int) * p->Ny) == hipSuccess &&
+              hipMalloc(&ctx->outd, sizeof(double) * 8) == hipSuccess &&
+              hipMalloc(&ctx->outi, sizeof(int) * 8) == hipSuccess &&
+              hipMalloc(&ctx->x0, sizeof(double) * p->Ny) == hipSuccess &&
+              hipHostMalloc(&ctx->h_pin, 64) == hipSuccess;
+    if (!ok) {
+        free_all(ctx);
+        delete ctx;
+        return RS_ERR_NOMEM;
+    }
+    // defined contents everywhere (ghosts, padding, unused latent-heat rows)
+    ok = hipMemsetAsync(ctx->psi[0], 0, fb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->psi[1], 0, fb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->c[0], 0, fb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->c[1], 0, fb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->T[0], 0, tb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->T[1], 0, tb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->lh, 0, lhb, ctx->stream) == hipSuccess &&
+         hipStreamSynchronize(ctx->stream) == hipSuccess;
+    if (!ok) {
+        free_all(ctx);
+        delete ctx;
+        return RS_ERR_HIP;
+    }
+    *out = ctx;
+    return RS_OK;
+}
+
+int rs_destroy(rs_ctx* ctx) {
+    if (!ctx) return RS_ERR_ARG;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+    return RS_OK;
+}
+
+const char* rs_last_error(const rs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rs_device_name(char* buf, int32_t len) {
+    if (!buf || len < 1) return RS_ERR_ARG;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return RS_ERR_HIP;
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount);
+    return RS_OK;
+}
+
+int rs_init_reference(rs_ctx* ctx) {
+    if (!ctx) return RS_ERR_ARG;
+    const rs_params& p = ctx->p;
+    std::vector<double> x0(p.Ny);
+    const double PI_REF = 3.1415927;  // fun.cu:59
+    for (int j = 0; j < p.Ny; ++j) x0[j] = p.i_tip_target + 0.5 * sin(2.0 * PI_REF * j / p.Ny);
+    return init_common(ctx, x0, 0);
+}
+
+int rs_init_config(rs_ctx* ctx, uint64_t seed) {
+    if (!ctx) return RS_ERR_ARG;
+    const rs_params& p = ctx->p;
+    std::vector<double> x0(p.Ny);
+    uint64_t st = seed;
+    for (int j = 1; j <= p.Ny - 2; ++j) {
+        const double u = (double)(splitmix64(&st) >> 11) * 0x1.0p-53;
+        x0[j] = p.i_tip_target + (2.0 * u - 1.0);
+    }
+    x0[0] = x0[p.Ny - 2];
+    x0[p.Ny - 1] = x0[1];
+    return init_common(ctx, x0, 1);
+}
+
+int rs_upload(rs_ctx* ctx, int32_t field, const double* host, int64_t count) {
+    if (!ctx || !host) return RS_ERR_ARG;
+    switch (field) {
+    case RS_FIELD_C:
+    case RS_FIELD_PSI:
+        if (count != ctx->n) return fail(ctx, RS_ERR_ARG, "field size mismatch");
+        HIPCHK(ctx, hipMemcpyAsync(field == RS_FIELD_C ? ctx->c[ctx->cur] : ctx->psi[ctx->cur],
+                                   host, sizeof(double) * count, hipMemcpyHostToDevice, ctx->stream));
+        break;
+    case RS_FIELD_T:
+        if (count != ctx->nT) return fail(ctx, RS_ERR_ARG, "T size mismatch");
+        HIPCHK(ctx, hipMemcpyAsync(ctx->T[0], host, sizeof(double) * count, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->T[1], host, sizeof(double) * count, hipMemcpyHostToDevice,
+                                   ctx->stream));
+        ctx->tcur = 0;
+        break;
+    default:
+        return fail(ctx, RS_ERR_ARG, "field cannot be uploaded");
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return RS_OK;
+}
+
+int rs_download(rs_ctx* ctx, int32_t field, double* host, int64_t count) {
+    if (!ctx || !host) return RS_ERR_ARG;
+    const int64_t need = (field == RS_FIELD_T) ? ctx->nT : ctx->n;
+    if (count != need) return fail(ctx, RS_ERR_ARG, "field size mismatch");
+    const double* src = nullptr;
+    switch (field) {
+    case RS_FIELD_C: src = ctx->c[ctx->cur]; break;
+    case RS_FIELD_PSI:
+    case RS_FIELD_PHI: src = ctx->psi[ctx->cur]; break;
+    case RS_FIELD_T: src = ctx->T[ctx->tcur]; break;
+    case RS_FIELD_TFINE: {
+        if (!ctx->tfine) HIPCHK(ctx, hipMalloc(&ctx->tfine, sizeof(double) * (size_t)ctx->n));
+        TempSrc ts = make_tempsrc(ctx->p, ctx->k, ctx->T[ctx->tcur], ctx->k.i_left_end_interp);
+        HIPCHK(ctx, launch_interp(ts, ctx->tfine, ctx->p.Nx, ctx->p.Ny, ctx->stream));
+        src = ctx->tfine;
+        break;
+    }
+    default: return fail(ctx, RS_ERR_ARG, "bad field");
+    }
+    HIPCHK(ctx, hipMemcpyAsync(host, src, sizeof(double) * count, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (field == RS_FIELD_PHI)
+        for (int64_t q = 0; q < count; ++q) host[q] = fmax(host[q], -1.0);
+    return RS_OK;
+}
+
+int rs_get_constants(rs_ctx* ctx, rs_constants* k) {
+    if (!ctx || !k) return RS_ERR_ARG;
+    *k = ctx->k;
+    return RS_OK;
+}
+
+int rs_set_constants(rs_ctx* ctx, const rs_constants* k) {
+    if (!ctx || !k) return RS_ERR_ARG;
+    if (k->NxT_all != ctx->k.NxT_all || k->NyT_all != ctx->k.NyT_all)
+        return fail(ctx, RS_ERR_ARG, "coarse grid geometry cannot change");
+    ctx->k = *k;
+    ctx->d = derive_constants(ctx->p, ctx->k.dt);
+    return RS_OK;
+}
+
+int rs_tip_column(rs_ctx* ctx, int32_t* i_tip) {
+    if (!ctx || !i_tip) return RS_ERR_ARG;
+    int it = 0;
+    int rc = do_tip_column(ctx, &it);
+    if (rc) return rc;
+    *i_tip = it;
+    return RS_OK;
+}
+
+int rs_pull_back(rs_ctx* ctx, int32_t i_offset) {
+    if (!ctx) return RS_ERR_ARG;
+    return do_pull_back(ctx, i_offset);
+}
+
+int rs_step(rs_ctx* ctx, int64_t nsteps) {
+    if (!ctx || nsteps < 0) return RS_ERR_ARG;
+    const rs_params& p = ctx->p;
+    rs_constants& k = ctx->k;
+    const double tau0 = tau0_of(p);
+    for (int64_t q = 0; q < nsteps; ++q) {
+        const int64_t step = k.step;
+        if (step % k.step_pull_back == 0) {  // fun.cu:448 (includes step 0)
+            int it = 0;
+            int rc = do_tip_column(ctx, &it);
+            if (rc) return rc;
+            rc = do_pull_back(ctx, it - p.i_tip_target);
+            if (rc) return rc;
+        }
+        if (ctx->timing) {
+            int rc = ev_record(ctx, 0);
+            if (rc) return rc;
+        }
+        const StepArgsF64 a = step_args(ctx);
+        HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+        if (ctx->timing) {
+            int rc = ev_record(ctx, 0);
+            if (rc) return rc;
+        }
+        if (p.ifTFC) {
+            TempArgs t;
+            t.lh = ctx->lh;
+            t.Nx = p.Nx;
+            t.NxT = k.NxT;
+            t.NyT = k.NyT;
+            t.NxT_all = k.NxT_all;
+            t.ratio = p.ratio_dxT_dx;
+            t.ile = k.i_left_end;
+            t.ratio_dt_dtT = k.ratio_dt_dtT;
+            t.vadv = vp_ramp(p, step * tau0 * k.dt) * tau0 / (p.S * p.W0);
+            t.dxT = p.dx * p.ratio_dxT_dx;
+            t.dtT = k.dtT;
+            t.DT = p.DT;
+            t.Gmu = p.Gamma * p.muk0;
+            t.Lcp = ctx->d.Lcp;
+            t.Tcold = k.Tcold;
+            t.Thot = k.Thot;
+            if (ctx->timing) {
+                int rc = ev_record(ctx, 1);
+                if (rc) return rc;
+            }
+            for (int sub = 0; sub < k.ratio_dt_dtT; ++sub) {
+                t.T = ctx->T[ctx->tcur];
+                t.Tn = ctx->T[ctx->tcur ^ 1];
+                HIPCHK(ctx, launch_temperature(t, ctx->stream));
+                ctx->tcur ^= 1;
+            }
+            if (ctx->timing) {
+                int rc = ev_record(ctx, 1);
+                if (rc) return rc;
+            }
+            k.i_left_end_interp = k.i_left_end;
+        }
+        ctx->cur ^= 1;
+        k.l_offset_Vp += vp_ramp(p, step * tau0 * k.dt) * k.dt * tau0 / (p.S * p.W0);
+        k.step = step + 1;
+    }
+    return RS_OK;
+}
+
+int rs_solute_mass(rs_ctx* ctx, double* mass) {
+    if (!ctx || !mass) return RS_ERR_ARG;
+    HIPCHK(ctx, launch_mass(ctx->c[ctx->cur], ctx->p.Nx, ctx->p.Ny, ctx->rowd, ctx->outd, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_pin, ctx->outd, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *mass = ctx->h_pin[0];
+    return RS_OK;
+}
+
+// set_tip_output (fun.cu:3024-3072) producing a tip.txt row
+// (src/main.cu:2671-2709)
+int rs_tip(rs_ctx* ctx, rs_tip_row* t) {
+    if (!ctx || !t) return RS_ERR_ARG;
+    const rs_params& p = ctx->p;
+    rs_constants& k = ctx->k;
+    std::vector<double> psi((size_t)ctx->n);
+    int rc = rs_download(ctx, RS_FIELD_PSI, psi.data(), ctx->n);
+    if (rc) return rc;
+    TipFit f;
+    const int st = precise_tip(p, psi.data(), &f);
+    const double tau0 = tau0_of(p);
+    const double dt = k.dt;
+    const double xnow = (k.i_offset_history + f.x) * p.dx;
+    double V = (xnow - k.x_tip_history) * p.S * p.W0 / (k.step_tip * tau0 * dt);
+    k.x_tip_history = xnow;
+    if (V < 0.0) V = 0.01;
+    if (V > 10.0) V = 10.0;
+    double Ttip, delta;
+    if (p.ifTFC) {
+        // the reference re-interpolates T_fine with the current i_left_end and
+        // samples row 0 (fun.cu:3052-3056)
+        k.i_left_end_interp = k.i_left_end;
+        std::vector<double> Tc((size_t)ctx->nT);
+        rc = rs_download(ctx, RS_FIELD_T, Tc.data(), ctx->nT);
+        if (rc) return rc;
+        const int r = p.ratio_dxT_dx;
+        const int i = f.i, j = 0, ile = k.i_left_end;
+        const double dxT = p.dx * r;
+        const int iT = (i - 1 + ile) / r;
+        const int jT = (int)((j - 1 + dxT) / r);
+        const double x = (i - 1.0 + ile) / r - iT;
+        const double y = (j - 1.0 + dxT) / r - jT;
+        const double* row0 = Tc.data() + (int64_t)jT * k.NxT_all;
+        const double* row1 = row0 + k.NxT_all;
+        const double f00 = row0[iT], f10 = row0[iT + 1], f01 = row1[iT], f11 = row1[iT + 1];
+        const double a10 = f10 - f00, a01 = f01 - f00, a11 = f11 - f10 - f01 + f00;
+        Ttip = f00 + a10 * x + a01 * y + a11 * x * y;
+        delta = (Ttip - p.Tmelt) / p.deltaT0;
+    } else {
+        delta = k.delta_initial +
+                p.GG * (f.i - p.i_tip_target + k.i_offset_history - k.i_offset_history_initial) * p.dx *
+                    p.S * p.W0 / p.deltaT0 -
+                p.GG * (k.l_offset_Vp - k.l_offset_Vp_initial) * p.S * p.W0 / p.deltaT0;
+        Ttip = delta * p.deltaT0 + p.Tmelt;
+    }
+    k.delta = delta;
+    if (k.if_begin_tip == 1) {
+        k.if_begin_tip = 0;
+        V = NAN;
+    }
+    t->step = k.step;
+    t->t_us = k.step * dt * tau0 * 1e-3;
+    t->delta = delta;
+    t->T_tip = Ttip;
+    t->V_tip = V;
+    t->Vp = vp_ramp(p, k.step * tau0 * dt);
+    t->R = f.R;
+    t->i = f.i;
+    t->j = f.j;
+    t->x = f.x;
+    t->y = f.y;
+    return st == 0 ? RS_OK : fail(ctx, RS_ERR_DOMAIN, "tip parabola fit failed");
+}
+
+int rs_sync(rs_ctx* ctx) {
+    if (!ctx) return RS_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return RS_OK;
+}
+
+int rs_timing_enable(rs_ctx* ctx, int32_t on) {
+    if (!ctx) return RS_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->timing = on != 0;
+    for (int q = 0; q < 3; ++q) ctx->nev[q] = 0;
+    return RS_OK;
+}
+
+int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count) {
+    if (!ctx || !avg_ms || !count || which < 0 || which > 1) return RS_ERR_ARG;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    int64_t nrec = 0;
+    for (int q = 0; q + 1 < ctx->nev[which]; q += 2) {
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[which][q], ctx->ev[which][q + 1]));
+        tot += ms;
+        ++nrec;
+    }
+    *avg_ms = nrec ? tot / nrec : 0.0;
+    *count = nrec;
+    return RS_OK;
+}
+
+}  // extern "C"

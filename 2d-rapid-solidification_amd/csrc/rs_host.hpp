@@ -1,0 +1,54 @@
+// rs_host.hpp -- host-side pieces of the library shared between its
+// translation units.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "rs_kernels.hpp"
+
+namespace rs {
+
+int common_tangent(const rs_params& p, double T, double* cl, double* cs);
+int liquidus(const rs_params& p, double c_liq, double* TL, double* c_sol);
+
+// Tip of the most advanced dendrite from a host psi array (Nx*Ny):
+// integer tip by the running row scan, then a quartic x(psi) fit on five
+// rows and a parabola x(y) through the five interface points
+// (get_precise_tip, fun.cu:2895-2954). Returns the parabola-fit status.
+struct TipFit {
+    int i, j;
+    double x, y, R;  // R in nm, clamped to [1, 1e5]
+};
+int precise_tip(const rs_params& p, const double* psi, TipFit* out);
+
+}  // namespace rs
+
+struct rs_ctx {
+    rs_params p;
+    rs_constants k;
+    rs::Derived d;
+    double orient[9];
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int64_t n = 0, nT = 0;
+    double* psi[2] = {nullptr, nullptr};
+    double* c[2] = {nullptr, nullptr};
+    double* T[2] = {nullptr, nullptr};
+    double* lh = nullptr;
+    double* rowd = nullptr;
+    int* rowi = nullptr;
+    double* outd = nullptr;
+    int* outi = nullptr;
+    double* x0 = nullptr;
+    double* tfine = nullptr;
+    double* h_pin = nullptr;  // pinned scalars for D2H
+    int cur = 0, tcur = 0;
+    int band_h = 40, wpb = 4;
+    std::string err;
+    // HIP-event timing of the fused step kernel [0], coarse T kernel [1] and
+    // the whole step [2]
+    bool timing = false;
+    std::vector<hipEvent_t> ev[3];
+    int nev[3] = {0, 0, 0};
+};

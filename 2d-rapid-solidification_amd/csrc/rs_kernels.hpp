@@ -1,0 +1,72 @@
+// rs_kernels.hpp -- launch-side declarations of the HIP kernels.
+#pragma once
+
+#include "rs_physics.hpp"
+
+namespace rs {
+
+// Column tile of the fused step kernel: one wavefront spans 64 consecutive
+// columns; lanes 1..62 own output columns, lanes 0 and 63 are the halo.
+constexpr int kWave = 64;
+constexpr int kTileOut = kWave - 2;
+
+// Fine-grid temperature source: coupled coarse grid (bilinear, fun.cu:1580)
+// or frozen gradient (get_delta, fun.cu:1610).
+struct TempSrc {
+    const double* T;  // coarse T (NxT_all pitch), TFC only
+    int tfc;
+    int ratio;        // ratio_dxT_dx
+    int NxT_all;
+    int ile;          // i_left_end used for the interpolation
+    double dxT;       // dx*ratio (y-index offset quirk, fun.cu:1590)
+    // frozen: delta(i) = (d0 + ((((GG*(i+ioff))*dx)*S)*W0)/dT0) - dl
+    int fta_ioff;
+    int S;
+    double d0, GG, dx, W0, dT0, dl, Tmelt;
+};
+
+struct StepArgsF64 {
+    const double* psi;
+    const double* c;
+    double* psi_n;
+    double* c_n;
+    double* lh;       // latent-heat column sums, (NyT-2) x Nx
+    int Nx, Ny;
+    int band_h;       // output rows per wavefront band (multiple of ratio)
+    int nbands;
+    int lh_last_row;  // last fine row that contributes latent heat, 10*(NyT-2)
+    int lh_on;
+    TempSrc ts;
+    double dt, AA, q_solid, dG_den, mu_den, kdc, dx2;
+    RK rk;
+    Aniso<double> an;
+};
+
+struct TempArgs {
+    const double* T;
+    double* Tn;
+    const double* lh;
+    int Nx, NxT, NyT, NxT_all, ratio, ile, ratio_dt_dtT;
+    double vadv;      // Vp_ramp(t) tau0 / (S W0)
+    double dxT, dtT, DT, Gmu /* Gamma*muk0 */, Lcp, Tcold, Thot;
+};
+
+hipError_t launch_step_fp64(const StepArgsF64& a, int waves_per_block, hipStream_t s);
+hipError_t launch_temperature(const TempArgs& a, hipStream_t s);
+hipError_t launch_init_T(double* T0, double* T1, int NxT_all, int NyT_all, double Tcold, int S,
+                         double W0, double dx, int r, double GG, hipStream_t s);
+hipError_t launch_init_fields(double* psi, double* c, const double* x0, int Nx, int Ny,
+                              double dx, int mode, double cinf, double c_liq, double c_sol,
+                              hipStream_t s);
+hipError_t launch_pull_back(const double* psi, const double* c, double* psi_n, double* c_n,
+                            int Nx, int Ny, int i_offset, double cinf, hipStream_t s);
+hipError_t launch_tip_column(const double* psi, int Nx, int Ny, double pin, int* rowbuf,
+                             int* out, hipStream_t s);
+hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* out,
+                       hipStream_t s);
+hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
+
+// host-side helpers that fill the temperature source from the run state
+TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tcoarse, int ile);
+
+}  // namespace rs

@@ -1,0 +1,209 @@
+"""Python handle on the HIP time-stepper library (librsolid.so, C ABI in
+include/rsolid.h).
+
+The library is the product: every field update runs in its HIP kernels.
+Importing this module loads the prebuilt shared library from this folder and
+fails loudly when it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (  # noqa: F401  (re-exported)
+    EXPORTED, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_STATE, RS_FIELD_C,
+    RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE, RS_OK, RS_PREC_FP64, RS_PREC_MIXED,
+    RsConstants, RsParams, RsTipRow,
+)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librsolid.so")
+
+
+class RsError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rsolid error {code}: {msg}")
+        self.code = code
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"HIP library not built: {LIB_PATH} is missing "
+            "(run python 2d-rapid-solidification_amd/build.py)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "rs_abi_info": (C.c_int, [P(C.c_int32)]),
+        "rs_params_reference": (C.c_int, [P(RsParams), C.c_int32, C.c_int32]),
+        "rs_params_config": (C.c_int, [P(RsParams), C.c_int32, C.c_int32, C.c_double]),
+        "rs_derive": (C.c_int, [P(RsParams), P(RsConstants)]),
+        "rs_create": (C.c_int, [P(RsParams), P(vp)]),
+        "rs_destroy": (C.c_int, [vp]),
+        "rs_last_error": (C.c_char_p, [vp]),
+        "rs_device_name": (C.c_int, [C.c_char_p, C.c_int32]),
+        "rs_init_reference": (C.c_int, [vp]),
+        "rs_init_config": (C.c_int, [vp, C.c_uint64]),
+        "rs_upload": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
+        "rs_download": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int64]),
+        "rs_get_constants": (C.c_int, [vp, P(RsConstants)]),
+        "rs_set_constants": (C.c_int, [vp, P(RsConstants)]),
+        "rs_step": (C.c_int, [vp, C.c_int64]),
+        "rs_tip_column": (C.c_int, [vp, P(C.c_int32)]),
+        "rs_pull_back": (C.c_int, [vp, C.c_int32]),
+        "rs_solute_mass": (C.c_int, [vp, P(C.c_double)]),
+        "rs_tip": (C.c_int, [vp, P(RsTipRow)]),
+        "rs_sync": (C.c_int, [vp]),
+        "rs_timing_enable": (C.c_int, [vp, C.c_int32]),
+        "rs_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def reference_params(Nx: int = 768, Ny: int = 832) -> RsParams:
+    """fun.cu values with the run.sh flags (fun.cu:20-174, run.sh:6-23)."""
+    p = RsParams()
+    rc = lib.rs_params_reference(C.byref(p), Nx, Ny)
+    if rc:
+        raise RsError(rc, "rs_params_reference")
+    return p
+
+
+def config_params(N: int, tfc: bool = False, alpha_ratio: float = 100.0) -> RsParams:
+    """Ti-10at%Nb configuration on an N x N grid (frozen or coupled T)."""
+    p = RsParams()
+    rc = lib.rs_params_config(C.byref(p), N, 1 if tfc else 0, alpha_ratio)
+    if rc:
+        raise RsError(rc, "rs_params_config")
+    return p
+
+
+def derive(p: RsParams) -> RsConstants:
+    k = RsConstants()
+    rc = lib.rs_derive(C.byref(p), C.byref(k))
+    if rc:
+        raise RsError(rc, "rs_derive")
+    return k
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    rc = lib.rs_device_name(buf, 256)
+    if rc:
+        raise RsError(rc, "rs_device_name")
+    return buf.value.decode()
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One time-stepper on the current HIP device (own stream and buffers)."""
+
+    def __init__(self, p: RsParams):
+        self.p = p.copy()
+        h = C.c_void_p()
+        rc = lib.rs_create(C.byref(self.p), C.byref(h))
+        if rc:
+            raise RsError(rc, "rs_create")
+        self._h = h
+        self.k0 = derive(self.p)
+
+    # -- lifetime -----------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            lib.rs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _chk(self, rc: int, what: str) -> None:
+        if rc:
+            raise RsError(rc, f"{what}: {lib.rs_last_error(self._h).decode()}")
+
+    # -- state ----------------------------------------------------------
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.p.Ny, self.p.Nx)
+
+    @property
+    def shape_T(self) -> tuple[int, int]:
+        return (self.k0.NyT_all, self.k0.NxT_all)
+
+    def init_reference(self) -> None:
+        self._chk(lib.rs_init_reference(self._h), "rs_init_reference")
+
+    def init_config(self, seed: int) -> None:
+        self._chk(lib.rs_init_config(self._h, C.c_uint64(seed)), "rs_init_config")
+
+    def upload(self, field: int, arr: np.ndarray) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        self._chk(lib.rs_upload(self._h, field, _dptr(a), a.size), "rs_upload")
+
+    def download(self, field: int) -> np.ndarray:
+        shp = self.shape_T if field == RS_FIELD_T else self.shape
+        out = np.empty(shp, dtype=np.float64)
+        self._chk(lib.rs_download(self._h, field, _dptr(out), out.size), "rs_download")
+        return out
+
+    def constants(self) -> RsConstants:
+        k = RsConstants()
+        self._chk(lib.rs_get_constants(self._h, C.byref(k)), "rs_get_constants")
+        return k
+
+    def set_constants(self, k: RsConstants) -> None:
+        self._chk(lib.rs_set_constants(self._h, C.byref(k)), "rs_set_constants")
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        self._chk(lib.rs_step(self._h, n), "rs_step")
+
+    def tip_column(self) -> int:
+        v = C.c_int32()
+        self._chk(lib.rs_tip_column(self._h, C.byref(v)), "rs_tip_column")
+        return v.value
+
+    def pull_back(self, i_offset: int) -> None:
+        self._chk(lib.rs_pull_back(self._h, i_offset), "rs_pull_back")
+
+    def solute_mass(self) -> float:
+        v = C.c_double()
+        self._chk(lib.rs_solute_mass(self._h, C.byref(v)), "rs_solute_mass")
+        return v.value
+
+    def tip(self) -> dict:
+        t = RsTipRow()
+        self._chk(lib.rs_tip(self._h, C.byref(t)), "rs_tip")
+        return t.as_dict()
+
+    def sync(self) -> None:
+        self._chk(lib.rs_sync(self._h), "rs_sync")
+
+    def timing_enable(self, on: bool = True) -> None:
+        self._chk(lib.rs_timing_enable(self._h, 1 if on else 0), "rs_timing_enable")
+
+    def timing(self, which: int) -> tuple[float, int]:
+        ms = C.c_double()
+        n = C.c_int64()
+        self._chk(lib.rs_timing(self._h, which, C.byref(ms), C.byref(n)), "rs_timing")
+        return ms.value, n.value

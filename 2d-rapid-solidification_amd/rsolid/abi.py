@@ -1,0 +1,88 @@
+"""ctypes mirror of include/rsolid.h (structs, enums, status codes).
+
+Field order and types must match the C header exactly; tests check the
+struct sizes against the values the library reports.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+RS_OK = 0
+RS_ERR_ARG = -1
+RS_ERR_HIP = -2
+RS_ERR_DOMAIN = -3
+RS_ERR_NOMEM = -4
+RS_ERR_STATE = -5
+
+RS_PREC_FP64 = 0
+RS_PREC_MIXED = 1
+
+RS_FIELD_C = 0
+RS_FIELD_PSI = 1
+RS_FIELD_PHI = 2
+RS_FIELD_T = 3
+RS_FIELD_TFINE = 4
+
+_I = C.c_int32
+_D = C.c_double
+
+
+class RsParams(C.Structure):
+    _fields_ = [
+        ("Nx", _I), ("Ny", _I), ("i_tip_target", _I), ("S", _I), ("ratio_dxT_dx", _I),
+        ("ifTFC", _I), ("if_Vp_ramp", _I), ("dt_cap", _I), ("num_fields_output", _I),
+        ("num_tip_output", _I), ("num_check_point_output", _I), ("precision", _I),
+        ("reserved0", _I),
+        ("W0", _D), ("Gamma", _D), ("muk0", _D), ("Tmelt", _D), ("deltaT0", _D), ("Dl", _D),
+        ("DT", _D), ("GG", _D), ("Vp", _D), ("Vp_ramp_rate", _D), ("eps1", _D), ("epk1", _D),
+        ("cinf", _D), ("latentHeat", _D), ("num_moles", _D), ("v0", _D), ("cp", _D), ("RR", _D),
+        ("dx", _D), ("AA", _D), ("TCOLD", _D), ("L0_bcc", _D), ("L0_liq", _D), ("L1_bcc", _D),
+        ("L1_liq", _D), ("dt_coef", _D), ("dtT_coef", _D), ("length_sample_um", _D),
+        ("lap_w_axis", _D), ("lap_w_diag", _D), ("q_solid", _D), ("total_time", _D),
+        ("delta_initial", _D), ("phi_interface", _D), ("c_liq0", _D), ("c_sol0", _D),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+    def copy(self) -> "RsParams":
+        q = RsParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(RsParams))
+        return q
+
+
+class RsConstants(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("initial_step", C.c_int64), ("total_step", C.c_int64),
+        ("step_fields", _I), ("step_tip", _I), ("step_check_point", _I), ("step_pull_back", _I),
+        ("if_begin_tip", _I), ("if_begin_field", _I), ("i_offset", _I),
+        ("i_offset_history_initial", _I), ("i_offset_history", _I), ("i_ignore", _I),
+        ("dt", _D), ("x_tip_history", _D), ("delta_initial", _D), ("delta", _D),
+        ("l_offset_Vp_initial", _D), ("l_offset_Vp", _D),
+        ("i_left_end_initial", _I), ("i_left_end", _I),
+        ("dtT", _D), ("ratio_dt_dtT", _I), ("NxT", _I), ("NyT", _I), ("NxT_all", _I),
+        ("NyT_all", _I), ("i_left_end_interp", _I),
+        ("Tcold", _D), ("Thot", _D), ("length_sample", _D),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class RsTipRow(C.Structure):
+    _fields_ = [
+        ("step", C.c_int64), ("t_us", _D), ("delta", _D), ("T_tip", _D), ("V_tip", _D),
+        ("Vp", _D), ("R", _D), ("i", _I), ("j", _I), ("x", _D), ("y", _D),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# every extern "C" symbol include/rsolid.h declares
+EXPORTED = [
+    "rs_abi_info", "rs_params_reference", "rs_params_config", "rs_derive", "rs_create", "rs_destroy",
+    "rs_last_error", "rs_device_name", "rs_init_reference", "rs_init_config", "rs_upload",
+    "rs_download", "rs_get_constants", "rs_set_constants", "rs_step", "rs_tip_column",
+    "rs_pull_back", "rs_solute_mass", "rs_tip", "rs_sync", "rs_timing_enable", "rs_timing",
+]

@@ -1,0 +1,138 @@
+"""Throughput benchmark of the coupled phi / c / T time-stepper.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--path fp64]
+
+One "step" is one pass of the reference loop body (fun.cu:448-509) over the
+whole N x N grid: fused phase-field/solute update, latent-heat block sums,
+coarse-grid heat equation and the pull-back check. Inputs are synthetic
+(the seeded planar Ti-10at%Nb front) and resident in HBM before timing.
+Prints ONE JSON line (rank 0).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "2d-rapid-solidification_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "MLUPS @4096² Ti-Nb: surrogate+FP32 vs plain FP64; tip-vel err ≤1%"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per cell update of the fused step kernel:
+# read + write of the persistent state psi and c (phi = max(psi,-1) is not
+# stored, fine T is rebuilt from the coarse grid)
+BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_step_fp64_4096.txt")
+
+
+def _traffic_from_profile():
+    """HBM bytes per launch of the fused kernel as measured by rocprofv3 PMC
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
+    recorded in profiles/; None when not measured."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
+def cpu_baseline(grid: int, seconds: float):
+    """The CPU oracle (C, FP64, OpenMP over rows) on the host cores, timed on
+    a bounded sample of the same workload: the first steps of the same grid."""
+    import oracle
+    import rsolid
+    cores = min(16, os.cpu_count() or 1)
+    oracle.set_threads(cores)
+    p = rsolid.config_params(grid, tfc=True)
+    st = oracle.State(p)
+    st.init_config(5)
+    st.step(1)                     # step 0 (pull-back) outside the timing
+    n, t0 = 0, time.perf_counter()
+    while True:
+        st.step(1)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    st.close()
+    return {"value": round(grid * grid * n / el / 1e6, 3), "unit": "MLUPS", "cores": cores,
+            "kind": "port",
+            "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--grid", type=int, default=4096)
+    ap.add_argument("--path", default="fp64", choices=["fp64"])
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the y-band decomposition is a design note, not code).", file=sys.stderr)
+        return 2
+
+    import rsolid
+    p = rsolid.config_params(a.grid, tfc=True)
+    ctx = rsolid.Context(p)
+    ctx.init_config(5)
+    ctx.step(a.warmup)
+    ctx.sync()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    ctx.step(a.steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    k_ms, k_n = ctx.timing(0)
+    t_ms, _ = ctx.timing(1)
+    ctx.timing_enable(False)
+    mass = ctx.solute_mass()
+    ctx.close()
+
+    cells = a.grid * a.grid
+    mlups = cells * a.steps / el / 1e6
+    bpc = BYTES_PER_CELL[a.path]
+    achieved = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(el * 1e3 / a.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded planar Ti-10at%Nb front (configuration preset), no dataset",
+        "config": {"workload": f"plain FP64 coupled phi/c/T step, {a.grid}x{a.grid} Ti-10at%Nb "
+                               "(coarse-grid heat equation, alpha = 100 D_l)",
+                   "grid": a.grid, "path": a.path, "coarse_T": [p.Nx, p.Ny]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic_from_profile(),
+                     "kernel": "step_fp64_kernel", "kernel_avg_ms": round(k_ms, 5),
+                     "kernel_launches": k_n, "bytes_per_cell": bpc},
+        "coarse_T_kernel_avg_ms": round(t_ms, 5),
+        "solute_mass_end": mass,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(a.grid, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,233 @@
+/*
+ * rsolid.h -- C ABI of the MI355X-native 2D rapid-solidification time-stepper
+ * (coupled phase field psi/phi, CALPHAD solute c, coarse-grid temperature T).
+ *
+ * The reference has no library boundary: its "interface" is the set of CUDA
+ * kernels that main() launches (src/main.hpp:54-85, fun.cu:431-514), the
+ * run-state struct `constants` (fun.cu:197-241, src/main.hpp:10-53) and the
+ * parameter set (src/constants.hpp:15-196 plus the -D flags of run.sh:75-90).
+ * This header replaces that surface with a coarse-grained, POD-only C ABI:
+ *
+ *   rs_params     <- constants.hpp names + fun.cu model macros      (inputs)
+ *   rs_constants  <- struct constants (fun.cu:197-241)              (run state)
+ *   rs_create/... <- main()'s cudaMalloc / set_parameters / set_initial
+ *   rs_step       <- one pass of the loop body fun.cu:448-509
+ *                    (pull_back, compute_phi, compute_psi, compute_c_CALPHAD,
+ *                     integrate_phi, compute_temperature x r,
+ *                     interpolate_temperature, swaps, l_offset_Vp update)
+ *
+ * Conventions: every call returns an int status (RS_OK == 0, negative on
+ * error) and never throws; rs_last_error() gives the message. Host arrays
+ * passed in are only read/written during the call and never retained.
+ * Field layout everywhere is the reference's: row-major pos(i,j) = j*Nx + i,
+ * x (i) = growth direction = contiguous axis, ghost cells included
+ * (fun.cu:176-179). One context per host thread; each context owns its own
+ * HIP stream and device buffers.
+ */
+#ifndef RSOLID_H
+#define RSOLID_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RS_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    RS_OK = 0,
+    RS_ERR_ARG = -1,      /* bad argument (null pointer, bad enum, bad size) */
+    RS_ERR_HIP = -2,      /* a HIP runtime call failed                       */
+    RS_ERR_DOMAIN = -3,   /* physical/geometric domain violated (T window)   */
+    RS_ERR_NOMEM = -4,    /* device allocation failed                        */
+    RS_ERR_STATE = -5     /* call not valid in the context's current state   */
+};
+
+/* ---- precision / thermodynamics paths ---------------------------------- */
+enum {
+    RS_PREC_FP64 = 0,     /* plain FP64 path: psi, c, T in FP64              */
+    RS_PREC_MIXED = 1     /* psi/phi and T in FP32; c, flux divergence,
+                             mass and ln-terms in FP64                        */
+};
+
+/* ---- fields for rs_upload / rs_download -------------------------------- */
+enum {
+    RS_FIELD_C = 0,       /* Ti mole fraction c, Nx*Ny                       */
+    RS_FIELD_PSI = 1,     /* psi (unclamped below -1), Nx*Ny                 */
+    RS_FIELD_PHI = 2,     /* phi = clamp(psi,-1,1), Nx*Ny (download only)    */
+    RS_FIELD_T = 3,       /* coarse temperature, NxT_all*NyT_all             */
+    RS_FIELD_TFINE = 4    /* fine temperature interpolated as the stepper
+                             sees it, Nx*Ny (download only)                  */
+};
+
+/*
+ * Model and material parameters. Field names follow src/constants.hpp and
+ * the fun.cu macros they replace (cited per field). Presets filling this
+ * struct live in the host library (rs_params_reference, rs_params_config).
+ */
+typedef struct rs_params {
+    /* grid: Nx_all, Ny_all (run.sh:6-7 -> -D Nx_all/Ny_all, fun.cu:38-39) */
+    int32_t Nx, Ny;
+    int32_t i_tip_target;      /* fun.cu:40 (Nx-268)                         */
+    int32_t S;                 /* SS, W/W0 (fun.cu:63)                        */
+    int32_t ratio_dxT_dx;      /* fun.cu:109 (10)                             */
+    int32_t ifTFC;             /* fun.cu:25: 1 coupled coarse T, 0 frozen T  */
+    int32_t if_Vp_ramp;        /* fun.cu:30                                   */
+    int32_t dt_cap;            /* fun.cu:1989-1991 cap dt at dt_coef*dx^2/4  */
+    int32_t num_fields_output;      /* fun.cu:50 */
+    int32_t num_tip_output;         /* fun.cu:51 */
+    int32_t num_check_point_output; /* fun.cu:52 */
+    int32_t precision;         /* RS_PREC_*                                   */
+    int32_t reserved0;
+
+    double W0;                 /* nm (fun.cu:64)                              */
+    double Gamma;              /* K*nm (fun.cu:65)                            */
+    double muk0;               /* nm/ns/K (fun.cu:68)                         */
+    double Tmelt;              /* K (fun.cu:72)                               */
+    double deltaT0;            /* K (fun.cu:73)                               */
+    double Dl;                 /* nm^2/ns (fun.cu:74)                         */
+    double DT;                 /* nm^2/ns thermal diffusivity (fun.cu:111)    */
+    double GG;                 /* K/nm (-D GG, run.sh:13)                     */
+    double Vp;                 /* m/s (-D Vp, run.sh:12)                      */
+    double Vp_ramp_rate;       /* m/s per us (fun.cu:1977, 0.0107587)         */
+    double eps1;               /* capillary anisotropy (-D eps1)              */
+    double epk1;               /* kinetic anisotropy (-D epk1)                */
+    double cinf;               /* far-field c, Ti fraction (fun.cu:62)        */
+    double latentHeat;         /* J/mol (fun.cu:87)                           */
+    double num_moles;          /* mol/m^3 (fun.cu:92)                         */
+    double v0;                 /* m^3/mol (fun.cu:91)                         */
+    double cp;                 /* J/mol/K (fun.cu:94)                         */
+    double RR;                 /* J/mol/K (fun.cu:103)                        */
+    double dx;                 /* W units (fun.cu:165-174)                    */
+    double AA;                 /* q(psi) parameter (fun.cu:165-174)           */
+    double TCOLD;              /* K (fun.cu:108)                              */
+    double L0_bcc, L0_liq;     /* J/mol Redlich-Kister L0 (fun.cu:1668-1669)  */
+    double L1_bcc, L1_liq;     /* J/mol Redlich-Kister L1 (0 = reference)     */
+    double dt_coef;            /* 0.6 (fun.cu:1989)                           */
+    double dtT_coef;           /* 0.9 (fun.cu:2029)                           */
+    double length_sample_um;   /* T sample length, um (fun.cu:2037, 50)       */
+    double lap_w_axis;         /* 2/3 (fun.cu:1051)                           */
+    double lap_w_diag;         /* 1/3 (fun.cu:1052); 1,0 = 5-point            */
+    double q_solid;            /* D_s/D_l floor of q(psi); 0 = reference      */
+    double total_time;         /* ns (-D total_time, run.sh:18)               */
+    double delta_initial;      /* -1 (fun.cu:2013)                            */
+    double phi_interface;      /* 0 (fun.cu:126)                              */
+    double c_liq0, c_sol0;     /* rs_init_config compositions (Ti fraction)   */
+} rs_params;
+
+/* Run state: field-for-field the reference's struct constants
+ * (fun.cu:197-241), plus i_left_end_interp (the i_left_end used by the last
+ * interpolate_temperature, i.e. the T_fine the next step reads). */
+typedef struct rs_constants {
+    int64_t step;
+    int64_t initial_step;
+    int64_t total_step;
+    int32_t step_fields;
+    int32_t step_tip;
+    int32_t step_check_point;
+    int32_t step_pull_back;
+    int32_t if_begin_tip;
+    int32_t if_begin_field;
+    int32_t i_offset;
+    int32_t i_offset_history_initial;
+    int32_t i_offset_history;
+    int32_t i_ignore;
+    double dt;
+    double x_tip_history;
+    double delta_initial;
+    double delta;
+    double l_offset_Vp_initial;
+    double l_offset_Vp;
+    int32_t i_left_end_initial;
+    int32_t i_left_end;
+    double dtT;
+    int32_t ratio_dt_dtT;
+    int32_t NxT;
+    int32_t NyT;
+    int32_t NxT_all;
+    int32_t NyT_all;
+    int32_t i_left_end_interp;
+    double Tcold;
+    double Thot;
+    double length_sample;
+} rs_constants;
+
+/* Tip record, one row of tip.txt (src/main.cu:2671-2709). */
+typedef struct rs_tip {
+    int64_t step;
+    double t_us;       /* (2)  */
+    double delta;      /* (3)  */
+    double T_tip;      /* (4)  K */
+    double V_tip;      /* (5)  m/s */
+    double Vp;         /* (6)  m/s */
+    double R;          /* (7)  nm */
+    int32_t i, j;      /* (8),(9) */
+    double x, y;       /* (10),(11) */
+} rs_tip_row;
+
+typedef struct rs_ctx rs_ctx;
+
+/* ---- parameters (host only, no device work) ---------------------------- */
+
+/* Reference preset: fun.cu macro values (fun.cu:20-174) with the run.sh -D
+ * flags (run.sh:6-23, 75-90) for the given grid. Replaces set_parameters'
+ * inputs. */
+int rs_params_reference(rs_params* p, int32_t Nx, int32_t Ny);
+
+/* Ti-10at%Nb configuration preset (frozen or coupled T) on an N x N grid.
+ * tfc = 0: frozen T(x,t) = T0 + G(x - V t); tfc = 1: coupled coarse-grid heat
+ * equation with thermal diffusivity DT = alpha_ratio * Dl. */
+int rs_params_config(rs_params* p, int32_t N, int32_t tfc, double alpha_ratio);
+
+/* Restatement of set_parameters (fun.cu:1984-2058): derives dt, cadences,
+ * coarse-T geometry, Tcold/Thot and i_left_end from the parameters. */
+int rs_derive(const rs_params* p, rs_constants* k);
+
+/* sizeof(rs_params), sizeof(rs_constants), sizeof(rs_tip_row) and
+ * RS_ABI_VERSION, for binding-side layout checks. No counterpart in the
+ * reference (it has no ABI). */
+int rs_abi_info(int32_t* sizes4);
+
+/* ---- context ------------------------------------------------------------ */
+int rs_create(const rs_params* p, rs_ctx** out);
+int rs_destroy(rs_ctx* ctx);
+const char* rs_last_error(const rs_ctx* ctx);   /* never NULL */
+int rs_device_name(char* buf, int32_t len);
+
+/* Initial conditions: initial + initial_temperature + interpolate_temperature
+ * (fun.cu:983-1028, 559-571). rs_init_config draws one front-position jitter
+ * in [-1,1) cells per row from splitmix64(seed). */
+int rs_init_reference(rs_ctx* ctx);
+int rs_init_config(rs_ctx* ctx, uint64_t seed);
+
+int rs_upload(rs_ctx* ctx, int32_t field, const double* host, int64_t count);
+int rs_download(rs_ctx* ctx, int32_t field, double* host, int64_t count);
+int rs_get_constants(rs_ctx* ctx, rs_constants* k);
+int rs_set_constants(rs_ctx* ctx, const rs_constants* k);
+
+/* Runs nsteps passes of the loop body fun.cu:448-509 starting at the
+ * context's current step: pull-back when step % step_pull_back == 0, the
+ * fused phi/psi/c update, latent heat, coarse T sub-steps, swaps and the
+ * l_offset_Vp update. Work is enqueued on the context stream; the call
+ * returns after the last pull-back decision (which needs the tip column). */
+int rs_step(rs_ctx* ctx, int64_t nsteps);
+
+/* Pieces of the loop body, for drivers that interleave their own output. */
+int rs_tip_column(rs_ctx* ctx, int32_t* i_tip);            /* get_itip_all_grain */
+int rs_pull_back(rs_ctx* ctx, int32_t i_offset);            /* set_pull_back+pull_back */
+int rs_solute_mass(rs_ctx* ctx, double* mass);              /* sum of c, interior */
+int rs_tip(rs_ctx* ctx, rs_tip_row* tip);                       /* set_tip_output */
+int rs_sync(rs_ctx* ctx);
+
+/* Average duration (ms) of the named kernel over the last rs_step call,
+ * measured with HIP events on the context stream. which: 0 = fused
+ * phi/psi/c step kernel, 1 = coarse T kernel, 2 = whole step. */
+int rs_timing_enable(rs_ctx* ctx, int32_t on);
+int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSOLID_H */

@@ -1,0 +1,211 @@
+"""ctypes handle on the CPU FP64 oracle (liboracle.so, oracle/rs_oracle.h).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+# struct layouts only (include/rsolid.h mirror); loading them this way keeps
+# the oracle independent of the HIP library
+_ABI = os.path.join(os.path.dirname(_HERE), "2d-rapid-solidification_amd", "rsolid", "abi.py")
+_spec = importlib.util.spec_from_file_location("rs_oracle_abi", _ABI)
+abi = importlib.util.module_from_spec(_spec)
+sys.modules["rs_oracle_abi"] = abi
+_spec.loader.exec_module(abi)
+RS_FIELD_T, RsConstants, RsParams, RsTipRow = abi.RS_FIELD_T, abi.RsConstants, abi.RsParams, abi.RsTipRow
+
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"oracle not built: {LIB_PATH} missing")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    d = C.c_double
+    sig = {
+        "or_derive": (C.c_int, [P(RsParams), P(RsConstants)]),
+        "or_tau0": (d, [P(RsParams)]),
+        "or_G_Ti_BCC": (d, [d]), "or_G_Ti_Liq": (d, [d]),
+        "or_G_Nb_BCC": (d, [d]), "or_G_Nb_Liq": (d, [d]),
+        "or_Gl": (d, [P(RsParams), d, d]), "or_Gs": (d, [P(RsParams), d, d]),
+        "or_Glc": (d, [P(RsParams), d, d]), "or_Gsc": (d, [P(RsParams), d, d]),
+        "or_anisotropy": (d, [P(RsParams), P(d), P(d), P(d), P(d)]),
+        "or_create": (vp, [P(RsParams)]),
+        "or_destroy": (None, [vp]),
+        "or_init_reference": (C.c_int, [vp]),
+        "or_init_config": (C.c_int, [vp, C.c_uint64]),
+        "or_get": (C.c_int, [vp, C.c_int, P(d), C.c_int64]),
+        "or_put": (C.c_int, [vp, C.c_int, P(d), C.c_int64]),
+        "or_get_constants": (C.c_int, [vp, P(RsConstants)]),
+        "or_set_constants": (C.c_int, [vp, P(RsConstants)]),
+        "or_step": (C.c_int, [vp, C.c_int64]),
+        "or_tip_column": (C.c_int, [vp]),
+        "or_pull_back": (C.c_int, [vp, C.c_int]),
+        "or_solute_mass": (d, [vp]),
+        "or_precise_tip": (C.c_int, [P(RsParams), P(d), P(C.c_int32), P(C.c_int32), P(d), P(d),
+                                      P(d)]),
+        "or_tip": (C.c_int, [vp, P(RsTipRow)]),
+        "or_common_tangent": (C.c_int, [P(RsParams), d, P(d), P(d)]),
+        "or_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
+        "or_set_threads": (C.c_int, [C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _cast(obj, cls):
+    """Same-layout struct from another module (e.g. rsolid.abi) -> cls."""
+    if isinstance(obj, cls):
+        return obj
+    out = cls()
+    if C.sizeof(obj) != C.sizeof(cls):
+        raise TypeError(f"struct size mismatch for {cls.__name__}")
+    C.memmove(C.byref(out), C.byref(obj), C.sizeof(cls))
+    return out
+
+
+def _P(p):
+    return C.byref(_cast(p, RsParams))
+
+
+def set_threads(n: int) -> int:
+    return lib.or_set_threads(n)
+
+
+def derive(p: RsParams) -> RsConstants:
+    k = RsConstants()
+    lib.or_derive(_P(p), C.byref(k))
+    return k
+
+
+def tau0(p: RsParams) -> float:
+    return lib.or_tau0(_P(p))
+
+
+def G(name: str, T: float) -> float:
+    return getattr(lib, "or_" + name)(T)
+
+
+def Gl(p, T, c): return lib.or_Gl(_P(p), T, c)
+def Gs(p, T, c): return lib.or_Gs(_P(p), T, c)
+def Glc(p, T, c): return lib.or_Glc(_P(p), T, c)
+def Gsc(p, T, c): return lib.or_Gsc(_P(p), T, c)
+
+
+def anisotropy(p: RsParams, ph3x3: np.ndarray, orient=None):
+    ph = np.ascontiguousarray(ph3x3, dtype=np.float64).reshape(9)
+    o = np.ascontiguousarray(orient if orient is not None else np.eye(3), dtype=np.float64).reshape(9)
+    a_s = C.c_double()
+    a_k = C.c_double()
+    v = lib.or_anisotropy(_P(p), _dptr(ph), _dptr(o), C.byref(a_s), C.byref(a_k))
+    return v, a_s.value, a_k.value
+
+
+def common_tangent(p: RsParams, T: float, cl0: float, cs0: float):
+    a = C.c_double(cl0)
+    b = C.c_double(cs0)
+    st = lib.or_common_tangent(_P(p), T, C.byref(a), C.byref(b))
+    return st, a.value, b.value
+
+
+def precise_tip(p: RsParams, psi: np.ndarray):
+    a = np.ascontiguousarray(psi, dtype=np.float64)
+    ti = C.c_int32()
+    tj = C.c_int32()
+    x = C.c_double()
+    y = C.c_double()
+    R = C.c_double()
+    st = lib.or_precise_tip(_P(p), _dptr(a), C.byref(ti), C.byref(tj), C.byref(x),
+                            C.byref(y), C.byref(R))
+    return st, dict(i=ti.value, j=tj.value, x=x.value, y=y.value, R=R.value)
+
+
+class State:
+    """The oracle's whole-run state (fields, coarse T, run constants)."""
+
+    def __init__(self, p: RsParams):
+        self.p = _cast(p, RsParams).copy()
+        self._h = lib.or_create(C.byref(self.p))
+        if not self._h:
+            raise RuntimeError("or_create failed")
+        self.k0 = derive(self.p)
+
+    def close(self):
+        if self._h:
+            lib.or_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *e):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise RuntimeError(f"oracle {what} failed: {rc}")
+
+    def init_reference(self):
+        self._chk(lib.or_init_reference(self._h), "init_reference")
+
+    def init_config(self, seed: int):
+        self._chk(lib.or_init_config(self._h, C.c_uint64(seed)), "init_config")
+
+    def get(self, field: int) -> np.ndarray:
+        shp = (self.k0.NyT_all, self.k0.NxT_all) if field == RS_FIELD_T else (self.p.Ny, self.p.Nx)
+        out = np.empty(shp, dtype=np.float64)
+        self._chk(lib.or_get(self._h, field, _dptr(out), out.size), "get")
+        return out
+
+    def put(self, field: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(lib.or_put(self._h, field, _dptr(a), a.size), "put")
+
+    def constants(self) -> RsConstants:
+        k = RsConstants()
+        lib.or_get_constants(self._h, C.byref(k))
+        return k
+
+    def set_constants(self, k: RsConstants):
+        lib.or_set_constants(self._h, C.byref(_cast(k, RsConstants)))
+
+    def step(self, n: int = 1):
+        self._chk(lib.or_step(self._h, n), "step")
+
+    def tip_column(self) -> int:
+        return lib.or_tip_column(self._h)
+
+    def pull_back(self, off: int):
+        self._chk(lib.or_pull_back(self._h, off), "pull_back")
+
+    def solute_mass(self) -> float:
+        return lib.or_solute_mass(self._h)
+
+    def tip(self) -> dict:
+        t = RsTipRow()
+        lib.or_tip(self._h, C.byref(t))
+        return t.as_dict()

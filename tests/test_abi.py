@@ -1,0 +1,64 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+include/rsolid.h declares, its struct layouts match the Python mirror, and
+its host-side parameter code agrees with the oracle's restatement."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from rsolid.abi import EXPORTED, RsConstants, RsParams, RsTipRow
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "rsolid.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rs_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported(rs):
+    decl = _declared()
+    assert sorted(EXPORTED) == decl
+    for name in decl:
+        assert hasattr(rs.lib, name), name
+
+
+def test_struct_layouts(rs):
+    s = (C.c_int32 * 4)()
+    assert rs.lib.rs_abi_info(s) == 0
+    assert s[0] == C.sizeof(RsParams)
+    assert s[1] == C.sizeof(RsConstants)
+    assert s[2] == C.sizeof(RsTipRow)
+    assert s[3] == 1
+
+
+@pytest.mark.parametrize("maker", ["ref768", "ref512", "cfg256", "cfg4096t", "cfg2048t"])
+def test_library_derive_matches_oracle(rs, oracle_mod, maker):
+    p = {"ref768": lambda: rs.reference_params(768, 832),
+         "ref512": lambda: rs.reference_params(512, 272),
+         "cfg256": lambda: rs.config_params(256, False),
+         "cfg4096t": lambda: rs.config_params(4096, True),
+         "cfg2048t": lambda: rs.config_params(2048, True)}[maker]()
+    a = rs.derive(p).as_dict()
+    b = oracle_mod.derive(p).as_dict()
+    assert a == b
+
+
+def test_config_preset_sits_on_the_liquidus(rs, oracle_mod):
+    p = rs.config_params(256, False)
+    st, cl, cs = oracle_mod.common_tangent(p, p.Tmelt, 0.9, 0.88)
+    assert st == 0
+    assert cl == pytest.approx(0.90, abs=1e-9)
+    assert cs == pytest.approx(p.c_sol0, abs=1e-9)
+    # the SGTE breakpoint 1941 K lies inside the frozen-T window of the run
+    assert p.Tmelt - p.deltaT0 < 1941.0 < p.Tmelt
+
+
+def test_bad_params_rejected_without_gpu(rs):
+    p = rs.reference_params(768, 832)
+    p.ratio_dxT_dx = 7
+    h = C.c_void_p()
+    assert rs.lib.rs_create(C.byref(p), C.byref(h)) == rs.RS_ERR_ARG
+    assert rs.lib.rs_params_reference(C.byref(p), 100, 100) == rs.RS_ERR_ARG
