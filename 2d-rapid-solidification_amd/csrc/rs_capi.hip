@@ -534,6 +534,27 @@ int rs_timing_enable(rs_ctx* ctx, int32_t on) {
     return RS_OK;
 }
 
+int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms) {
+    if (!ctx || !avg_ms || n < 1 || n > ctx->n || reps < 1) return RS_ERR_ARG;
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    double tot = 0.0;
+    for (int q = 0; q < reps; ++q) {
+        HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+        HIPCHK(ctx, launch_copy(ctx->psi[0], ctx->psi[1], n, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+        HIPCHK(ctx, hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+        tot += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = tot / reps;
+    return RS_OK;
+}
+
 int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count) {
     if (!ctx || !avg_ms || !count || which < 0 || which > 1) return RS_ERR_ARG;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

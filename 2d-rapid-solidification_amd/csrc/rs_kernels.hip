@@ -396,6 +396,21 @@ hipError_t launch_interp(const TempSrc& ts, double* Tf, int Nx, int Ny, hipStrea
     return hipGetLastError();
 }
 
+// plain streaming copy (profiler byte-counter calibration)
+__global__ __launch_bounds__(256) void copy_kernel(const double* __restrict__ a,
+                                                   double* __restrict__ b, int64_t n) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; q < n; q += stride) b[q] = a[q];
+}
+
+hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
 TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tc, int ile) {
     TempSrc t;
     t.T = Tc;

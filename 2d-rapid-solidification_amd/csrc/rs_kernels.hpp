@@ -65,6 +65,7 @@ hipError_t launch_tip_column(const double* psi, int Nx, int Ny, double pin, int*
 hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* out,
                        hipStream_t s);
 hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
+hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s);
 
 // host-side helpers that fill the temperature source from the run state
 TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tcoarse, int ile);

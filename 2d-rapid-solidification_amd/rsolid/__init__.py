@@ -58,6 +58,7 @@ def _load() -> C.CDLL:
         "rs_sync": (C.c_int, [vp]),
         "rs_timing_enable": (C.c_int, [vp, C.c_int32]),
         "rs_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
+        "rs_calibrate_copy": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -201,6 +202,12 @@ class Context:
 
     def timing_enable(self, on: bool = True) -> None:
         self._chk(lib.rs_timing_enable(self._h, 1 if on else 0), "rs_timing_enable")
+
+    def calibrate_copy(self, n: int, reps: int = 5) -> float:
+        """Average ms of a plain FP64 copy of n doubles (destroys the fields)."""
+        ms = C.c_double()
+        self._chk(lib.rs_calibrate_copy(self._h, n, reps, C.byref(ms)), "rs_calibrate_copy")
+        return ms.value
 
     def timing(self, which: int) -> tuple[float, int]:
         ms = C.c_double()

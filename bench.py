@@ -28,7 +28,7 @@ HBM_PEAK_GBS = 8000.0
 # read + write of the persistent state psi and c (phi = max(psi,-1) is not
 # stored, fine T is rebuilt from the coarse grid)
 BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_step_fp64_4096.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_fp64_kernel_4096.md")
 
 
 def _traffic_from_profile():
@@ -77,6 +77,9 @@ def main() -> int:
     ap.add_argument("--path", default="fp64", choices=["fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
+                         "(byte-counter calibration for rocprofv3 --pmc)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -98,6 +101,8 @@ def main() -> int:
     t_ms, _ = ctx.timing(1)
     ctx.timing_enable(False)
     mass = ctx.solute_mass()
+    kc = ctx.constants()
+    copy_ms = ctx.calibrate_copy(a.grid * a.grid, 5) if a.calibrate_copy else None
     ctx.close()
 
     cells = a.grid * a.grid
@@ -119,7 +124,7 @@ def main() -> int:
         "data": "synthetic: seeded planar Ti-10at%Nb front (configuration preset), no dataset",
         "config": {"workload": f"plain FP64 coupled phi/c/T step, {a.grid}x{a.grid} Ti-10at%Nb "
                                "(coarse-grid heat equation, alpha = 100 D_l)",
-                   "grid": a.grid, "path": a.path, "coarse_T": [p.Nx, p.Ny]},
+                   "grid": a.grid, "path": a.path, "coarse_T": [kc.NxT, kc.NyT]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": _traffic_from_profile(),
@@ -128,6 +133,10 @@ def main() -> int:
         "coarse_T_kernel_avg_ms": round(t_ms, 5),
         "solute_mass_end": mass,
     }
+    if copy_ms is not None:
+        nb = 2 * 8 * a.grid * a.grid
+        out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(copy_ms, 5),
+                                   "GBps": round(nb / (copy_ms * 1e-3) / 1e9, 1)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.grid, a.cpu_seconds)
     print(json.dumps(out), flush=True)

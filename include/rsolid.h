@@ -227,6 +227,13 @@ int rs_sync(rs_ctx* ctx);
 int rs_timing_enable(rs_ctx* ctx, int32_t on);
 int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
 
+/* Measurement helper with no counterpart in the reference: launches a plain
+ * FP64 copy kernel (n doubles read, n written, n <= Nx*Ny) reps times on
+ * the context stream between the context's two psi buffers, so profiler
+ * byte counters can be calibrated against a known traffic. Destroys the
+ * field contents; writes the average kernel time (HIP events). */
+int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms);
+
 #ifdef __cplusplus
 }
 #endif
