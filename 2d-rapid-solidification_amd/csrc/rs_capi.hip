@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 
 #include "rs_host.hpp"
@@ -66,6 +67,9 @@ void free_all(rs_ctx* ctx) {
                     ctx->lh, ctx->rowd, ctx->outd, ctx->x0, ctx->tfine};
     for (double* d : ds)
         if (d) (void)hipFree(d);
+    float* fs[] = {ctx->psi32[0], ctx->psi32[1], ctx->T32[0], ctx->T32[1]};
+    for (float* f : fs)
+        if (f) (void)hipFree(f);
     if (ctx->rowi) (void)hipFree(ctx->rowi);
     if (ctx->outi) (void)hipFree(ctx->outi);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
@@ -134,20 +138,94 @@ int do_pull_back(rs_ctx* ctx, int off) {
     if (ctx->p.ifTFC)
         k.i_left_end = k.i_left_end_initial + k.i_offset_history - (int)(k.l_offset_Vp / ctx->p.dx);
     const int c0 = ctx->cur;
-    HIPCHK(ctx, launch_pull_back(ctx->psi[c0], ctx->c[c0], ctx->psi[c0 ^ 1], ctx->c[c0 ^ 1],
-                                 ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf, ctx->stream));
+    if (ctx->mixed)
+        HIPCHK(ctx, launch_pull_back_f32(ctx->psi32[c0], ctx->c[c0], ctx->psi32[c0 ^ 1],
+                                         ctx->c[c0 ^ 1], ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf,
+                                         ctx->stream));
+    else
+        HIPCHK(ctx, launch_pull_back(ctx->psi[c0], ctx->c[c0], ctx->psi[c0 ^ 1], ctx->c[c0 ^ 1],
+                                     ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf, ctx->stream));
     ctx->cur ^= 1;
     return check_window(ctx, k.i_left_end);
 }
 
 int do_tip_column(rs_ctx* ctx, int* it) {
-    HIPCHK(ctx, launch_tip_column(ctx->psi[ctx->cur], ctx->p.Nx, ctx->p.Ny, ctx->p.phi_interface,
-                                  ctx->rowi, ctx->outi, ctx->stream));
+    if (ctx->mixed)
+        HIPCHK(ctx, launch_tip_column_f32(ctx->psi32[ctx->cur], ctx->p.Nx, ctx->p.Ny,
+                                          (float)ctx->p.phi_interface, ctx->rowi, ctx->outi,
+                                          ctx->stream));
+    else
+        HIPCHK(ctx, launch_tip_column(ctx->psi[ctx->cur], ctx->p.Nx, ctx->p.Ny, ctx->p.phi_interface,
+                                      ctx->rowi, ctx->outi, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_pin, ctx->outi, sizeof(int), hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     memcpy(it, ctx->h_pin, sizeof(int));
     return RS_OK;
+}
+
+// FP32 copies of both coarse T buffers (mixed path), spare row included
+int refresh_T32(rs_ctx* ctx) {
+    if (!ctx->mixed) return RS_OK;
+    const int64_t n = ctx->nT + ctx->k.NxT_all;
+    for (int q = 0; q < 2; ++q)
+        HIPCHK(ctx, launch_T32_copy(ctx->T[q], ctx->T32[q], n, ctx->poly.Tref, ctx->stream));
+    return RS_OK;
+}
+
+StepArgsMixed step_args_mixed(rs_ctx* ctx) {
+    const rs_params& p = ctx->p;
+    const rs_constants& k = ctx->k;
+    StepArgsMixed a;
+    a.psi = ctx->psi32[ctx->cur];
+    a.c = ctx->c[ctx->cur];
+    a.psi_n = ctx->psi32[ctx->cur ^ 1];
+    a.c_n = ctx->c[ctx->cur ^ 1];
+    a.lh = ctx->lh;
+    a.T32 = ctx->T32[ctx->tcur];
+    a.Nx = p.Nx;
+    a.Ny = p.Ny;
+    a.band_h = ctx->band_h;
+    a.nbands = (p.Ny - 2 + a.band_h - 1) / a.band_h;
+    a.lh_last_row = p.ratio_dxT_dx * (k.NyT - 2);
+    a.lh_on = p.ifTFC;
+    a.tfc = p.ifTFC;
+    a.ratio = p.ratio_dxT_dx;
+    a.NxT_all = k.NxT_all;
+    a.ile = k.i_left_end_interp;
+    a.dxT = (float)(p.dx * p.ratio_dxT_dx);
+    // frozen T: T(i) = Tmelt + dT0 (d0 - dl) + GG dx S W0 (i + ioff), as T - Tref
+    const double gstep = p.GG * p.dx * p.S * p.W0;
+    const int ioff = -p.i_tip_target + k.i_offset_history - k.i_offset_history_initial;
+    const double dl = p.GG * (k.l_offset_Vp - k.l_offset_Vp_initial) * p.S * p.W0 / p.deltaT0;
+    a.fta_a = p.Tmelt + p.deltaT0 * (k.delta_initial - dl) + gstep * ioff - ctx->poly.Tref;
+    a.fta_b = gstep;
+    a.dt = (float)k.dt;
+    a.AA = (float)p.AA;
+    a.q_solid = (float)p.q_solid;
+    a.dG_den_inv = (float)(1.0 / ctx->d.dG_den);
+    a.eps1 = (float)p.eps1;
+    a.epk1 = (float)p.epk1;
+    a.lap_w_axis = (float)p.lap_w_axis;
+    a.lap_w_diag = (float)p.lap_w_diag;
+    a.inv2dx = (float)(1.0 / (2.0 * p.dx));
+    a.invdx2 = (float)(1.0 / (p.dx * p.dx));
+    a.inv4dx2 = (float)(1.0 / (4.0 * p.dx * p.dx));
+    a.inv2dx2 = (float)(1.0 / (2.0 * p.dx * p.dx));
+    const int oi[6] = {0, 1, 3, 4, 6, 7};
+    for (int q = 0; q < 6; ++q) a.ops[q] = (float)ctx->orient[oi[q]];
+    a.mu_den = ctx->d.mu_den;
+    a.kdc_dx2 = ctx->d.kdc / (p.dx * p.dx);
+    a.L0l = p.L0_liq;
+    a.L0s = p.L0_bcc;
+    a.L1l = p.L1_liq;
+    a.L1s = p.L1_bcc;
+    a.RR = p.RR;
+    a.dL0 = (float)(p.L0_liq - p.L0_bcc);
+    a.dL1 = (float)(p.L1_liq - p.L1_bcc);
+    a.poly = ctx->poly;
+    a.rk = rk_of(p);
+    return a;
 }
 
 int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
@@ -159,12 +237,18 @@ int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
     orientation(ctx->orient, 0.0, 0.0, 0.0);
     HIPCHK(ctx, hipMemcpyAsync(ctx->x0, x0.data(), sizeof(double) * p.Ny, hipMemcpyHostToDevice,
                                ctx->stream));
-    HIPCHK(ctx, launch_init_fields(ctx->psi[0], ctx->c[0], ctx->x0, p.Nx, p.Ny, p.dx, mode, p.cinf,
-                                   p.c_liq0, p.c_sol0, ctx->stream));
+    if (ctx->mixed)
+        HIPCHK(ctx, launch_init_fields_f32(ctx->psi32[0], ctx->c[0], ctx->x0, p.Nx, p.Ny, p.dx,
+                                           mode, p.cinf, p.c_liq0, p.c_sol0, ctx->stream));
+    else
+        HIPCHK(ctx, launch_init_fields(ctx->psi[0], ctx->c[0], ctx->x0, p.Nx, p.Ny, p.dx, mode,
+                                       p.cinf, p.c_liq0, p.c_sol0, ctx->stream));
     if (p.ifTFC) {
         // NyT_all rows plus the spare row the bilinear read may touch
         HIPCHK(ctx, launch_init_T(ctx->T[0], ctx->T[1], ctx->k.NxT_all, ctx->k.NyT_all + 1,
                                   ctx->k.Tcold, p.S, p.W0, p.dx, p.ratio_dxT_dx, p.GG, ctx->stream));
+        int rc = refresh_T32(ctx);
+        if (rc) return rc;
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->k.i_left_end_interp = ctx->k.i_left_end;
@@ -225,8 +309,29 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     // row jT+1 reaches row NyT_all, with weight 0, when NyT % 4 == 0)
     const size_t tb = sizeof(double) * (size_t)(ctx->nT + ctx->k.NxT_all);
     const size_t lhb = sizeof(double) * (size_t)p->Nx * (size_t)(ctx->k.NyT > 2 ? ctx->k.NyT - 2 : 1);
-    bool ok = hipMalloc(&ctx->psi[0], fb) == hipSuccess && hipMalloc(&ctx->psi[1], fb) == hipSuccess &&
-              hipMalloc(&ctx->c[0], fb) == hipSuccess && hipMalloc(&ctx->c[1], fb) == hipSuccess &&
+    ctx->mixed = p->precision == RS_PREC_MIXED;
+    const size_t fb32 = sizeof(float) * (size_t)ctx->n;
+    const size_t tb32 = sizeof(float) * (size_t)(ctx->nT + ctx->k.NxT_all);
+    bool ok = true;
+    if (ctx->mixed) {
+        ok = hipMalloc(&ctx->psi32[0], fb32) == hipSuccess &&
+             hipMalloc(&ctx->psi32[1], fb32) == hipSuccess &&
+             hipMalloc(&ctx->T32[0], tb32) == hipSuccess && hipMalloc(&ctx->T32[1], tb32) == hipSuccess &&
+             hipMemsetAsync(ctx->psi32[0], 0, fb32, ctx->stream) == hipSuccess &&
+             hipMemsetAsync(ctx->psi32[1], 0, fb32, ctx->stream) == hipSuccess;
+        // SGTE temperature window of the run: frozen T spans the grid and drifts
+        // with the pulling; coupled T spans Tcold..Thot plus recalescence
+        const double T0 = p->Tmelt - p->deltaT0;
+        const double span = p->GG * p->dx * p->S * p->W0 * p->Nx;
+        double lo = p->ifTFC ? std::min(ctx->k.Tcold, T0) : T0 - span;
+        double hi = p->ifTFC ? std::max(ctx->k.Thot, p->Tmelt) : T0 + span;
+        ctx->poly_err = fit_poly_table(*p, lo - 100.0, hi + 100.0, T0, &ctx->poly);
+    } else {
+        ok = hipMalloc(&ctx->psi[0], fb) == hipSuccess && hipMalloc(&ctx->psi[1], fb) == hipSuccess &&
+             hipMemsetAsync(ctx->psi[0], 0, fb, ctx->stream) == hipSuccess &&
+             hipMemsetAsync(ctx->psi[1], 0, fb, ctx->stream) == hipSuccess;
+    }
+    ok = ok && hipMalloc(&ctx->c[0], fb) == hipSuccess && hipMalloc(&ctx->c[1], fb) == hipSuccess &&
               hipMalloc(&ctx->T[0], tb) == hipSuccess && hipMalloc(&ctx->T[1], tb) == hipSuccess &&
               hipMalloc(&ctx->lh, lhb) == hipSuccess &&
               hipMalloc(&ctx->rowd, sizeof(double) * p->Ny) == hipSuccess &&
@@ -241,9 +346,7 @@ int rs_create(const rs_params* p, rs_ctx** out) {
         return RS_ERR_NOMEM;
     }
     // defined contents everywhere (ghosts, padding, unused latent-heat rows)
-    ok = hipMemsetAsync(ctx->psi[0], 0, fb, ctx->stream) == hipSuccess &&
-         hipMemsetAsync(ctx->psi[1], 0, fb, ctx->stream) == hipSuccess &&
-         hipMemsetAsync(ctx->c[0], 0, fb, ctx->stream) == hipSuccess &&
+    ok = hipMemsetAsync(ctx->c[0], 0, fb, ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->c[1], 0, fb, ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->T[0], 0, tb, ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->T[1], 0, tb, ctx->stream) == hipSuccess &&
@@ -255,6 +358,18 @@ int rs_create(const rs_params* p, rs_ctx** out) {
         return RS_ERR_HIP;
     }
     *out = ctx;
+    return RS_OK;
+}
+
+int rs_context_info(rs_ctx* ctx, double* out, int32_t n) {
+    if (!ctx || !out || n < 1) return RS_ERR_ARG;
+    const double v[6] = {ctx->poly_err,
+                         ctx->mixed ? ctx->poly.Tref : 0.0,
+                         ctx->mixed ? ctx->poly.Tref + ctx->poly.lo : 0.0,
+                         ctx->mixed ? ctx->poly.Tref + ctx->poly.hi : 0.0,
+                         (double)ctx->band_h,
+                         (double)ctx->wpb};
+    for (int q = 0; q < n && q < 6; ++q) out[q] = v[q];
     return RS_OK;
 }
 
@@ -304,21 +419,33 @@ int rs_init_config(rs_ctx* ctx, uint64_t seed) {
 
 int rs_upload(rs_ctx* ctx, int32_t field, const double* host, int64_t count) {
     if (!ctx || !host) return RS_ERR_ARG;
+    std::vector<float> f32;
     switch (field) {
     case RS_FIELD_C:
     case RS_FIELD_PSI:
         if (count != ctx->n) return fail(ctx, RS_ERR_ARG, "field size mismatch");
-        HIPCHK(ctx, hipMemcpyAsync(field == RS_FIELD_C ? ctx->c[ctx->cur] : ctx->psi[ctx->cur],
-                                   host, sizeof(double) * count, hipMemcpyHostToDevice, ctx->stream));
+        if (field == RS_FIELD_PSI && ctx->mixed) {
+            f32.resize((size_t)count);
+            for (int64_t q = 0; q < count; ++q) f32[q] = (float)host[q];
+            HIPCHK(ctx, hipMemcpyAsync(ctx->psi32[ctx->cur], f32.data(), sizeof(float) * count,
+                                       hipMemcpyHostToDevice, ctx->stream));
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(field == RS_FIELD_C ? ctx->c[ctx->cur] : ctx->psi[ctx->cur],
+                                       host, sizeof(double) * count, hipMemcpyHostToDevice,
+                                       ctx->stream));
+        }
         break;
-    case RS_FIELD_T:
+    case RS_FIELD_T: {
         if (count != ctx->nT) return fail(ctx, RS_ERR_ARG, "T size mismatch");
         HIPCHK(ctx, hipMemcpyAsync(ctx->T[0], host, sizeof(double) * count, hipMemcpyHostToDevice,
                                    ctx->stream));
         HIPCHK(ctx, hipMemcpyAsync(ctx->T[1], host, sizeof(double) * count, hipMemcpyHostToDevice,
                                    ctx->stream));
         ctx->tcur = 0;
+        int rc = refresh_T32(ctx);
+        if (rc) return rc;
         break;
+    }
     default:
         return fail(ctx, RS_ERR_ARG, "field cannot be uploaded");
     }
@@ -331,6 +458,15 @@ int rs_download(rs_ctx* ctx, int32_t field, double* host, int64_t count) {
     const int64_t need = (field == RS_FIELD_T) ? ctx->nT : ctx->n;
     if (count != need) return fail(ctx, RS_ERR_ARG, "field size mismatch");
     const double* src = nullptr;
+    if ((field == RS_FIELD_PSI || field == RS_FIELD_PHI) && ctx->mixed) {
+        std::vector<float> f32((size_t)count);
+        HIPCHK(ctx, hipMemcpyAsync(f32.data(), ctx->psi32[ctx->cur], sizeof(float) * count,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (int64_t q = 0; q < count; ++q)
+            host[q] = field == RS_FIELD_PHI ? fmax((double)f32[q], -1.0) : (double)f32[q];
+        return RS_OK;
+    }
     switch (field) {
     case RS_FIELD_C: src = ctx->c[ctx->cur]; break;
     case RS_FIELD_PSI:
@@ -399,8 +535,13 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             int rc = ev_record(ctx, 0);
             if (rc) return rc;
         }
-        const StepArgsF64 a = step_args(ctx);
-        HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+        if (ctx->mixed) {
+            const StepArgsMixed a = step_args_mixed(ctx);
+            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ctx->stream));
+        } else {
+            const StepArgsF64 a = step_args(ctx);
+            HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+        }
         if (ctx->timing) {
             int rc = ev_record(ctx, 0);
             if (rc) return rc;
@@ -423,6 +564,7 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             t.Lcp = ctx->d.Lcp;
             t.Tcold = k.Tcold;
             t.Thot = k.Thot;
+            t.Tref = ctx->poly.Tref;
             if (ctx->timing) {
                 int rc = ev_record(ctx, 1);
                 if (rc) return rc;
@@ -430,6 +572,7 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             for (int sub = 0; sub < k.ratio_dt_dtT; ++sub) {
                 t.T = ctx->T[ctx->tcur];
                 t.Tn = ctx->T[ctx->tcur ^ 1];
+                t.T32n = ctx->mixed ? ctx->T32[ctx->tcur ^ 1] : nullptr;
                 HIPCHK(ctx, launch_temperature(t, ctx->stream));
                 ctx->tcur ^= 1;
             }
@@ -542,7 +685,7 @@ int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms) {
     double tot = 0.0;
     for (int q = 0; q < reps; ++q) {
         HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
-        HIPCHK(ctx, launch_copy(ctx->psi[0], ctx->psi[1], n, ctx->stream));
+        HIPCHK(ctx, launch_copy(ctx->c[0], ctx->c[1], n, ctx->stream));
         HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
         HIPCHK(ctx, hipEventSynchronize(e1));
         float ms = 0.f;

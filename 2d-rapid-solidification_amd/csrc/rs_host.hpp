@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "rs_kernels.hpp"
+#include "rs_mixed.hpp"
 
 namespace rs {
 
@@ -35,6 +36,12 @@ struct rs_ctx {
     double* psi[2] = {nullptr, nullptr};
     double* c[2] = {nullptr, nullptr};
     double* T[2] = {nullptr, nullptr};
+    // mixed-precision path: psi in FP32 and an FP32 copy of T - Tref
+    float* psi32[2] = {nullptr, nullptr};
+    float* T32[2] = {nullptr, nullptr};
+    rs::PolyTable poly;
+    double poly_err = 0.0;
+    bool mixed = false;
     double* lh = nullptr;
     double* rowd = nullptr;
     int* rowi = nullptr;

@@ -192,7 +192,9 @@ __global__ __launch_bounds__(256) void temperature_kernel(TempArgs a) {
     if (iT >= a.NxT || jT >= a.NyT) return;
     const int64_t P = a.NxT_all;
     if (iT == 0 || iT == a.NxT - 1) {
-        a.Tn[jT * P + iT] = (iT == 0) ? a.Tcold : a.Thot;
+        const double v = (iT == 0) ? a.Tcold : a.Thot;
+        a.Tn[jT * P + iT] = v;
+        if (a.T32n) a.T32n[jT * P + iT] = (float)(v - a.Tref);
         return;
     }
     if (jT == 0 || jT == a.NyT - 1) return;
@@ -222,6 +224,12 @@ __global__ __launch_bounds__(256) void temperature_kernel(TempArgs a) {
     a.Tn[jT * P + iT] = tn;
     if (jT == a.NyT - 2) a.Tn[iT] = tn;                      // row 0 <- row NyT-2
     if (jT == 1) a.Tn[(int64_t)(a.NyT - 1) * P + iT] = tn;  // row NyT-1 <- row 1
+    if (a.T32n) {
+        const float t32 = (float)(tn - a.Tref);
+        a.T32n[jT * P + iT] = t32;
+        if (jT == a.NyT - 2) a.T32n[iT] = t32;
+        if (jT == 1) a.T32n[(int64_t)(a.NyT - 1) * P + iT] = t32;
+    }
 }
 
 hipError_t launch_temperature(const TempArgs& a, hipStream_t s) {

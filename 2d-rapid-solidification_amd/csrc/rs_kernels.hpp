@@ -49,6 +49,8 @@ struct TempArgs {
     int Nx, NxT, NyT, NxT_all, ratio, ile, ratio_dt_dtT;
     double vadv;      // Vp_ramp(t) tau0 / (S W0)
     double dxT, dtT, DT, Gmu /* Gamma*muk0 */, Lcp, Tcold, Thot;
+    float* T32n;      // optional FP32 copy of T_next - Tref (mixed path), or null
+    double Tref;
 };
 
 hipError_t launch_step_fp64(const StepArgsF64& a, int waves_per_block, hipStream_t s);

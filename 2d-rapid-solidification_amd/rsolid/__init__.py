@@ -59,6 +59,7 @@ def _load() -> C.CDLL:
         "rs_timing_enable": (C.c_int, [vp, C.c_int32]),
         "rs_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "rs_calibrate_copy": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double)]),
+        "rs_context_info": (C.c_int, [vp, P(C.c_double), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -79,12 +80,14 @@ def reference_params(Nx: int = 768, Ny: int = 832) -> RsParams:
     return p
 
 
-def config_params(N: int, tfc: bool = False, alpha_ratio: float = 100.0) -> RsParams:
+def config_params(N: int, tfc: bool = False, alpha_ratio: float = 100.0,
+                  precision: int = RS_PREC_FP64) -> RsParams:
     """Ti-10at%Nb configuration on an N x N grid (frozen or coupled T)."""
     p = RsParams()
     rc = lib.rs_params_config(C.byref(p), N, 1 if tfc else 0, alpha_ratio)
     if rc:
         raise RsError(rc, "rs_params_config")
+    p.precision = precision
     return p
 
 
@@ -202,6 +205,12 @@ class Context:
 
     def timing_enable(self, on: bool = True) -> None:
         self._chk(lib.rs_timing_enable(self._h, 1 if on else 0), "rs_timing_enable")
+
+    def info(self) -> dict:
+        v = (C.c_double * 6)()
+        self._chk(lib.rs_context_info(self._h, v, 6), "rs_context_info")
+        return {"poly_err": v[0], "Tref": v[1], "T_lo": v[2], "T_hi": v[3],
+                "band_h": int(v[4]), "waves_per_block": int(v[5])}
 
     def calibrate_copy(self, n: int, reps: int = 5) -> float:
         """Average ms of a plain FP64 copy of n doubles (destroys the fields)."""

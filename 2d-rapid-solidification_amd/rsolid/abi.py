@@ -85,5 +85,5 @@ EXPORTED = [
     "rs_last_error", "rs_device_name", "rs_init_reference", "rs_init_config", "rs_upload",
     "rs_download", "rs_get_constants", "rs_set_constants", "rs_step", "rs_tip_column",
     "rs_pull_back", "rs_solute_mass", "rs_tip", "rs_sync", "rs_timing_enable", "rs_timing",
-    "rs_calibrate_copy",
+    "rs_calibrate_copy", "rs_context_info",
 ]

@@ -193,6 +193,13 @@ int rs_abi_info(int32_t* sizes4);
 /* ---- context ------------------------------------------------------------ */
 int rs_create(const rs_params* p, rs_ctx** out);
 int rs_destroy(rs_ctx* ctx);
+
+/* Context facts with no counterpart in the reference (diagnostics):
+ * out[0] = largest error (J/mol) of the mixed path's SGTE polynomial table
+ *          against the FP64 routine on its window (0 on the FP64 path),
+ * out[1] = Tref, out[2] = window low (K), out[3] = window high (K),
+ * out[4] = band height (rows per wavefront), out[5] = waves per block. */
+int rs_context_info(rs_ctx* ctx, double* out, int32_t n);
 const char* rs_last_error(const rs_ctx* ctx);   /* never NULL */
 int rs_device_name(char* buf, int32_t len);
 
