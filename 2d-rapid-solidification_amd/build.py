@@ -58,8 +58,21 @@ def _compile(src: str) -> str:
     return obj
 
 
+def gen_sgte_table() -> str:
+    """Build and run the host generator of the mixed stepper's SGTE
+    polynomial and log tables (csrc/rs_sgte_poly.inc)."""
+    os.makedirs(BUILD, exist_ok=True)
+    exe = os.path.join(BUILD, "gen_sgte_poly")
+    out = os.path.join(CSRC, "rs_sgte_poly.inc")
+    _run([HIPCC, "-O2", "-std=c++17", "-ffp-contract=off", os.path.join(CSRC, "gen_sgte_poly.cpp"),
+          "-o", exe])
+    _run([exe, out])
+    return out
+
+
 def build_hip(jobs: int = 4) -> str:
     os.makedirs(BUILD, exist_ok=True)
+    gen_sgte_table()
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB, *objs])

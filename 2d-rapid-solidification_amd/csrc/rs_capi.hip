@@ -44,6 +44,12 @@ int check_params(const rs_params& p, std::string* why) {
     if (p.precision != RS_PREC_FP64 && p.precision != RS_PREC_MIXED) { *why = "bad precision"; return RS_ERR_ARG; }
     if (p.ifTFC && p.Ny < 22) { *why = "coupled T needs Ny >= 22"; return RS_ERR_ARG; }
     if (p.i_tip_target < 1 || p.i_tip_target > p.Nx - 2) { *why = "i_tip_target outside the grid"; return RS_ERR_ARG; }
+    if ((int64_t)p.Nx * p.Ny >= (int64_t)1 << 31) { *why = "grid exceeds 2^31 cells (32-bit offsets)"; return RS_ERR_ARG; }
+    if (p.precision == RS_PREC_MIXED) {
+        // the mixed stepper indexes the coarse-T rows with integer arithmetic
+        const double dxT = p.dx * p.ratio_dxT_dx;
+        if (fabs(dxT - (double)lround(dxT)) > 1e-12) { *why = "mixed path needs dx*ratio_dxT_dx integral"; return RS_ERR_ARG; }
+    }
     return RS_OK;
 }
 
@@ -169,7 +175,7 @@ int refresh_T32(rs_ctx* ctx) {
     if (!ctx->mixed) return RS_OK;
     const int64_t n = ctx->nT + ctx->k.NxT_all;
     for (int q = 0; q < 2; ++q)
-        HIPCHK(ctx, launch_T32_copy(ctx->T[q], ctx->T32[q], n, ctx->poly.Tref, ctx->stream));
+        HIPCHK(ctx, launch_T32_copy(ctx->T[q], ctx->T32[q], n, kMixedTref, ctx->stream));
     return RS_OK;
 }
 
@@ -190,15 +196,15 @@ StepArgsMixed step_args_mixed(rs_ctx* ctx) {
     a.lh_last_row = p.ratio_dxT_dx * (k.NyT - 2);
     a.lh_on = p.ifTFC;
     a.tfc = p.ifTFC;
-    a.ratio = p.ratio_dxT_dx;
     a.NxT_all = k.NxT_all;
     a.ile = k.i_left_end_interp;
-    a.dxT = (float)(p.dx * p.ratio_dxT_dx);
+    a.dxTi = (int)lround(p.dx * p.ratio_dxT_dx);
+    a.has_L1 = (p.L1_liq != 0.0 || p.L1_bcc != 0.0) ? 1 : 0;
     // frozen T: T(i) = Tmelt + dT0 (d0 - dl) + GG dx S W0 (i + ioff), as T - Tref
     const double gstep = p.GG * p.dx * p.S * p.W0;
     const int ioff = -p.i_tip_target + k.i_offset_history - k.i_offset_history_initial;
     const double dl = p.GG * (k.l_offset_Vp - k.l_offset_Vp_initial) * p.S * p.W0 / p.deltaT0;
-    a.fta_a = p.Tmelt + p.deltaT0 * (k.delta_initial - dl) + gstep * ioff - ctx->poly.Tref;
+    a.fta_a = p.Tmelt + p.deltaT0 * (k.delta_initial - dl) + gstep * ioff - kMixedTref;
     a.fta_b = gstep;
     a.dt = (float)k.dt;
     a.AA = (float)p.AA;
@@ -214,16 +220,12 @@ StepArgsMixed step_args_mixed(rs_ctx* ctx) {
     a.inv2dx2 = (float)(1.0 / (2.0 * p.dx * p.dx));
     const int oi[6] = {0, 1, 3, 4, 6, 7};
     for (int q = 0; q < 6; ++q) a.ops[q] = (float)ctx->orient[oi[q]];
-    a.mu_den = ctx->d.mu_den;
-    a.kdc_dx2 = ctx->d.kdc / (p.dx * p.dx);
+    a.K = ctx->d.kdc / (p.dx * p.dx) * ctx->d.mu_den;
     a.L0l = p.L0_liq;
-    a.L0s = p.L0_bcc;
     a.L1l = p.L1_liq;
-    a.L1s = p.L1_bcc;
     a.RR = p.RR;
     a.dL0 = (float)(p.L0_liq - p.L0_bcc);
     a.dL1 = (float)(p.L1_liq - p.L1_bcc);
-    a.poly = ctx->poly;
     a.rk = rk_of(p);
     return a;
 }
@@ -319,13 +321,6 @@ int rs_create(const rs_params* p, rs_ctx** out) {
              hipMalloc(&ctx->T32[0], tb32) == hipSuccess && hipMalloc(&ctx->T32[1], tb32) == hipSuccess &&
              hipMemsetAsync(ctx->psi32[0], 0, fb32, ctx->stream) == hipSuccess &&
              hipMemsetAsync(ctx->psi32[1], 0, fb32, ctx->stream) == hipSuccess;
-        // SGTE temperature window of the run: frozen T spans the grid and drifts
-        // with the pulling; coupled T spans Tcold..Thot plus recalescence
-        const double T0 = p->Tmelt - p->deltaT0;
-        const double span = p->GG * p->dx * p->S * p->W0 * p->Nx;
-        double lo = p->ifTFC ? std::min(ctx->k.Tcold, T0) : T0 - span;
-        double hi = p->ifTFC ? std::max(ctx->k.Thot, p->Tmelt) : T0 + span;
-        ctx->poly_err = fit_poly_table(*p, lo - 100.0, hi + 100.0, T0, &ctx->poly);
     } else {
         ok = hipMalloc(&ctx->psi[0], fb) == hipSuccess && hipMalloc(&ctx->psi[1], fb) == hipSuccess &&
              hipMemsetAsync(ctx->psi[0], 0, fb, ctx->stream) == hipSuccess &&
@@ -363,11 +358,9 @@ int rs_create(const rs_params* p, rs_ctx** out) {
 
 int rs_context_info(rs_ctx* ctx, double* out, int32_t n) {
     if (!ctx || !out || n < 1) return RS_ERR_ARG;
-    const double v[6] = {ctx->poly_err,
-                         ctx->mixed ? ctx->poly.Tref : 0.0,
-                         ctx->mixed ? ctx->poly.Tref + ctx->poly.lo : 0.0,
-                         ctx->mixed ? ctx->poly.Tref + ctx->poly.hi : 0.0,
-                         (double)ctx->band_h,
+    double lo = 0.0, hi = 0.0, err = 0.0;
+    if (ctx->mixed) sgte_table_info(&lo, &hi, &err);
+    const double v[6] = {err, ctx->mixed ? kMixedTref : 0.0, lo, hi, (double)ctx->band_h,
                          (double)ctx->wpb};
     for (int q = 0; q < n && q < 6; ++q) out[q] = v[q];
     return RS_OK;
@@ -537,7 +530,10 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
         }
         if (ctx->mixed) {
             const StepArgsMixed a = step_args_mixed(ctx);
-            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ctx->stream));
+            const double* o = ctx->orient;
+            const bool ident = o[0] == 1.0 && o[1] == 0.0 && o[3] == 0.0 && o[4] == 1.0 &&
+                               o[6] == 0.0 && o[7] == 0.0;
+            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->stream));
         } else {
             const StepArgsF64 a = step_args(ctx);
             HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
@@ -564,7 +560,7 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             t.Lcp = ctx->d.Lcp;
             t.Tcold = k.Tcold;
             t.Thot = k.Thot;
-            t.Tref = ctx->poly.Tref;
+            t.Tref = kMixedTref;
             if (ctx->timing) {
                 int rc = ev_record(ctx, 1);
                 if (rc) return rc;

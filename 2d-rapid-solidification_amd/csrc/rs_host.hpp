@@ -39,8 +39,6 @@ struct rs_ctx {
     // mixed-precision path: psi in FP32 and an FP32 copy of T - Tref
     float* psi32[2] = {nullptr, nullptr};
     float* T32[2] = {nullptr, nullptr};
-    rs::PolyTable poly;
-    double poly_err = 0.0;
     bool mixed = false;
     double* lh = nullptr;
     double* rowd = nullptr;

@@ -1,43 +1,90 @@
 // rs_mixed.hip -- mixed-precision fused step for gfx950.
 //
 // Same wavefront-marching layout as the FP64 stepper (62 output columns per
-// wavefront, a band of rows per wavefront, 3-row register window, cross-lane
-// shuffles), with:
-//   * psi (hence phi) stored and updated in FP32, anisotropy in FP32 with
-//     reciprocal/rsqrt arithmetic;
-//   * fine T rebuilt in FP32 as T' = T - Tref from an FP32 copy of the
+// wavefront, a band of rows per wavefront, 3-row register window), with:
+//   * psi (hence phi) stored and updated in FP32; anisotropy in FP32 with
+//     reciprocal/rsqrt arithmetic (specialised for the identity orientation);
+//   * fine T rebuilt in FP32 as T' = T - 1941 K from an FP32 copy of the
 //     coarse grid, which itself is advanced in FP64;
-//   * SGTE temperature dependence from the two-piece polynomial table (FP32
-//     Horner, FP64 constant term), ln(c/(1-c)) and the chemical potential
-//     assembled in FP64; cells outside the table window use the exact FP64
-//     routine;
-//   * conservative face fluxes in FP64: each face flux is computed once, by
-//     one lane, and used with opposite signs by the two cells it separates,
-//     so the interior scheme conserves solute to rounding.
+//   * SGTE temperature dependence from a build-time polynomial table
+//     (rs_sgte_poly.inc: 100 K pieces, degree 5, FP32 Horner with FP64
+//     constant terms, coefficients as instruction immediates), chosen per
+//     wavefront with a uniform loop over the pieces present; cells outside
+//     the table use the exact FP64 routine;
+//   * ln(c/(1-c)) in FP64 from a 128-entry table log (Tang's method: table
+//     value ln F_k plus a degree-6 log1p of the reduced argument) and a
+//     Newton-refined FP64 reciprocal, instead of an IEEE divide and libm log;
+//   * conservative face fluxes: FP32 mobility weights, FP64 chemical
+//     potential differences, fluxes and divergence; each face flux is
+//     computed once, by one lane, and used with opposite signs by the two
+//     cells it separates, so the interior scheme conserves solute to
+//     rounding;
+//   * i+-1 neighbours through DPP wavefront shifts (wave_shr:1 / wave_shl:1).
 // The file is compiled without floating-point contraction: every FMA is
 // written out, so that the prologue of a band and the loop body of the band
 // above it produce bitwise-identical face fluxes.
 #include <math.h>
 
-#include <algorithm>
-#include <vector>
-
 #include "rs_mixed.hpp"
+#include "rs_sgte_poly.inc"
 
 namespace rs {
 
-template <typename T>
-__device__ __forceinline__ T shup(T v) {
-    return __shfl_up(v, 1, kWave);
+// lane l <- lane l-1 / lane l+1; the lane with no source gets 0 (it is a
+// halo lane whose value is never used)
+__device__ __forceinline__ int dpp_up(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
 }
-template <typename T>
-__device__ __forceinline__ T shdn(T v) {
-    return __shfl_down(v, 1, kWave);
+__device__ __forceinline__ int dpp_dn(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float shup(float v) { return __int_as_float(dpp_up(__float_as_int(v))); }
+__device__ __forceinline__ float shdn(float v) { return __int_as_float(dpp_dn(__float_as_int(v))); }
+__device__ __forceinline__ double shup(double v) {
+    return __hiloint2double(dpp_up(__double2hiint(v)), dpp_up(__double2loint(v)));
+}
+__device__ __forceinline__ double shdn(double v) {
+    return __hiloint2double(dpp_dn(__double2hiint(v)), dpp_dn(__double2loint(v)));
 }
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// fine temperature relative to Tref, FP32
+// ln r for normal r > 0, FP64, from the LDS table tab[2k] = 1/F_k,
+// tab[2k+1] = ln F_k with F_k = 1 + (k + 1/2)/128 (error ~2 ulp)
+__device__ __forceinline__ double ln_tab(double r, const double* tab) {
+    int e;
+    const double m = frexp(r, &e);       // m in [0.5, 1)
+    const double m2 = m + m;              // [1, 2), exact
+    const int hi = __double2hiint(m2);
+    const int k = (hi >> 13) & 127;       // top 7 mantissa bits
+    const double F = __hiloint2double((hi & (int)0xFFFFE000) | 0x1000, 0);
+    const double d = m2 - F;              // exact, |d| <= 2^-8
+    const double u = d * tab[2 * k];
+    double q = fma(u, -1.0 / 6.0, 0.2);
+    q = fma(u, q, -0.25);
+    q = fma(u, q, 1.0 / 3.0);
+    q = fma(u, q, -0.5);
+    const double l1p = fma(u * u, q, u);  // log1p(u), |u| < 2^-8
+    const double ef = (double)(e - 1);
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    return fma(ef, ln2_hi, tab[2 * k + 1]) + fma(ef, ln2_lo, l1p);
+}
+
+// c/(1-c) with a Newton-refined reciprocal (~1 ulp)
+__device__ __forceinline__ double ratio_c(double c, double om) {
+    double y = __builtin_amdgcn_rcp(om);
+    y = fma(y, fma(-om, y, 1.0), y);
+    double r = c * y;
+    r = fma(fma(-om, r, c), y, r);
+    return r;
+}
+
+// coarse-T rows and the y weight for fine row j (uniform over the wavefront);
+// dx*ratio is an integer number of cells, so the reference's
+// int((j-1+dxT)/10) is exact integer arithmetic here. The coarse/fine ratio
+// (ratio_dxT_dx, fun.cu:109) is fixed at 10 by the library.
+constexpr int kRatio = 10;
+
 struct RowT {
     const float* r0;
     const float* r1;
@@ -46,19 +93,19 @@ struct RowT {
 
 __device__ __forceinline__ RowT row_t(const StepArgsMixed& a, int j) {
     RowT rt;
-    const int jT = (int)((j - 1 + (double)a.dxT) / a.ratio);
-    const double y = (j - 1.0 + (double)a.dxT) / a.ratio - jT;
-    rt.r0 = a.T32 + (int64_t)jT * a.NxT_all;
+    const int g = j - 1 + a.dxTi;
+    const int jT = g / kRatio;
+    rt.r0 = a.T32 + jT * a.NxT_all;
     rt.r1 = rt.r0 + a.NxT_all;
-    rt.y = (float)y;
+    rt.y = (float)(g - kRatio * jT) * 0.1f;
     return rt;
 }
 
 __device__ __forceinline__ float temp_m(const StepArgsMixed& a, const RowT& rt, int i) {
     if (a.tfc) {
         const int g = i - 1 + a.ile;
-        const int iT = g / a.ratio;
-        const float x = (float)(g - a.ratio * iT) * (1.0f / 10.0f);
+        const int iT = g / kRatio;
+        const float x = (float)(g - kRatio * iT) * 0.1f;
         const float f00 = rt.r0[iT], f10 = rt.r0[iT + 1];
         const float f01 = rt.r1[iT], f11 = rt.r1[iT + 1];
         const float a10 = f10 - f00, a01 = f01 - f00, a11 = (f11 - f10) - (f01 - f00);
@@ -67,86 +114,106 @@ __device__ __forceinline__ float temp_m(const StepArgsMixed& a, const RowT& rt, 
     return (float)fma(a.fta_b, (double)i, a.fta_a);
 }
 
-template <int PIECE>
-__device__ __forceinline__ void eval_poly(const PolyTable& P, float Tp, float v[4]) {
-    const float t = (Tp - P.cen[PIECE]) * P.isc[PIECE];
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-        float acc = P.c[PIECE][f][kPolyDeg - 1];
-#pragma unroll
-        for (int k = kPolyDeg - 2; k >= 0; --k) acc = fmaf(acc, t, P.c[PIECE][f][k]);
-        v[f] = acc * t;  // sum_{k>=1} c_k t^k
-    }
+// table piece of T' (-1 outside the table)
+__device__ __forceinline__ int sgte_piece(float Tp) {
+    if (!(Tp >= RS_SGTE_T0 && Tp < RS_SGTE_T1)) return -1;
+    return min((int)((Tp - RS_SGTE_T0) * 0.01f), RS_SGTE_PIECES - 1);
 }
 
-// per-cell thermodynamics: FP32 driving force dG = f_L - f_S and FP64
-// chemical potential mu = mu_L + h (mu_S - mu_L)
-__device__ __forceinline__ void thermo_m(const StepArgsMixed& a, float Tp, double c, float h,
-                                         float& dG, double& mu) {
-    const PolyTable& P = a.poly;
-    const double onemc = 1.0 - c;
-    double lr = 0.0;
-    if (c > 1.e-8 && onemc > 1.e-8) lr = log(c / onemc);
-    if (Tp >= P.lo && Tp < P.hi) {
-        float v[4];
-        double c0[4];
-        if (Tp < P.tb) {
-            eval_poly<0>(P, Tp, v);
-#pragma unroll
-            for (int f = 0; f < 4; ++f) c0[f] = P.c0[0][f];
-        } else {
-            eval_poly<1>(P, Tp, v);
-#pragma unroll
-            for (int f = 0; f < 4; ++f) c0[f] = P.c0[1][f];
-        }
+struct Uni {
+    float vU, vTi, vNb;   // non-constant parts (t^1..t^5) of U_L, dG_Ti, dG_Nb
+    double c0U;
+    float c0Ti, c0Nb;
+};
+
+#define RS_H5(t, c1, c2, c3, c4, c5) \
+    ((t) * fmaf(fmaf(fmaf(fmaf((c5), (t), (c4)), (t), (c3)), (t), (c2)), (t), (c1)))
+
+#define RS_SGTE_CASE(P, CEN, ISC, K0, K1, K2, a1, a2, a3, a4, a5, b1, b2, b3, b4, b5, d1, d2, d3, \
+                     d4, d5)                                                                      \
+    case P: {                                                                                     \
+        const float t = (Tp - (CEN)) * (ISC);                                                     \
+        u.vU = RS_H5(t, a1, a2, a3, a4, a5);                                                      \
+        u.vTi = RS_H5(t, b1, b2, b3, b4, b5);                                                     \
+        u.vNb = RS_H5(t, d1, d2, d3, d4, d5);                                                     \
+        u.c0U = (K0);                                                                             \
+        u.c0Ti = (float)(K1);                                                                     \
+        u.c0Nb = (float)(K2);                                                                     \
+        break;                                                                                    \
+    }
+
+// evaluates the piece of the wavefront's first lane (uniform: immediates);
+// lanes in another piece (only wavefronts straddling a piece boundary
+// isotherm) are flagged for the exact routine
+__device__ __forceinline__ Uni sgte_eval(float Tp, int& piece) {
+    Uni u{0.f, 0.f, 0.f, 0.0, 0.f, 0.f};
+    const int pp = __builtin_amdgcn_readfirstlane(piece);
+    if (piece != pp) piece = -1;
+    switch (pp) { RS_SGTE_TABLE(RS_SGTE_CASE) default: break; }
+    return u;
+}
+
+// per-cell thermodynamics: FP32 driving force dG = f_L - f_S (the ideal
+// mixing terms cancel between the phases) and the FP64 chemical potential
+// mu = mu_L + h (mu_S - mu_L)
+__device__ __forceinline__ void thermo_m(const StepArgsMixed& a, const double* lntab, float Tp,
+                                         double c, float h, float& dG, double& mu) {
+    int piece = sgte_piece(Tp);
+    const Uni u = sgte_eval(Tp, piece);
+    if (piece >= 0) {
+        const double onemc = 1.0 - c;
+        double lr = 0.0;
+        if (c > 1.e-8 && onemc > 1.e-8) lr = ln_tab(ratio_c(c, onemc), lntab);
         const float cf = (float)c;
         const float omcf = (float)onemc;
         const float om2 = 1.0f - 2.0f * cf;
-        const float dGTi = (float)c0[2] + v[2];
-        const float dGNb = (float)c0[3] + v[3];
-        const float ex = fmaf(a.dL1, om2, a.dL0);
-        dG = fmaf(cf * omcf, ex, fmaf(cf, dGTi, omcf * dGNb));
+        const float dGTi = u.c0Ti + u.vTi;
+        const float dGNb = u.c0Nb + u.vNb;
         const double om2d = 1.0 - 2.0 * c;
-        const double six = 1.0 - 6.0 * c + 6.0 * (c * c);
-        const double mu64 = c0[0] + a.L0l * om2d + a.L1l * six + a.RR * (P.Tref + (double)Tp) * lr;
-        const float dsl = (float)(c0[1] - c0[0]) + (v[1] - v[0]) +
-                          fmaf(a.dL0, -om2, -a.dL1 * (float)six);
-        // mu_S - mu_L = (U_S - U_L) + (L0s - L0l)(1-2c) + (L1s - L1l)(..)
-        const float mu32 = fmaf(h, dsl, v[0]);
-        mu = mu64 + (double)mu32;
+        double mu64 = u.c0U + a.L0l * om2d + a.RR * (kMixedTref + (double)Tp) * lr;
+        // mu_S - mu_L = (dG_Nb - dG_Ti) - dL0 (1-2c) [- dL1 (1-6c+6c^2)]
+        float dsl = (dGNb - dGTi) - a.dL0 * om2;
+        float ex = a.dL0;
+        if (a.has_L1) {
+            const double sixd = 1.0 - 6.0 * c + 6.0 * (c * c);
+            mu64 += a.L1l * sixd;
+            dsl -= a.dL1 * (float)sixd;
+            ex = fmaf(a.dL1, om2, ex);
+        }
+        dG = fmaf(cf * omcf, ex, fmaf(cf, dGTi, omcf * dGNb));
+        mu = mu64 + (double)fmaf(h, dsl, u.vU);
     } else {
-        const double T = P.Tref + (double)Tp;
-        const Thermo th = thermo_fp64(a.rk, T, c);
+        const Thermo th = thermo_fp64(a.rk, kMixedTref + (double)Tp, c);
         dG = (float)th.dG;
         mu = th.muL + (double)h * (th.muS - th.muL);
     }
 }
 
 struct CellM {
-    float psi, phi, dG;
-    double c, S, mu;
+    float psi, phi, dG, S;
+    double c, mu;
 };
 
-__device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const RowT& rt, float psi,
-                                             double c, int i) {
+__device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const double* lntab,
+                                             const RowT& rt, float psi, double c, int i) {
     CellM q;
     q.psi = psi;
     q.c = c;
     q.phi = fmaxf(psi, -1.0f);
-    const float qv = qq<float>(a.AA, a.q_solid, psi);
-    q.S = (double)qv * (c * (1.0 - c));
+    const float cf = (float)c;
+    q.S = qq<float>(a.AA, a.q_solid, psi) * (cf * (1.0f - cf));
     const float Tp = temp_m(a, rt, i);
-    const float g = gg<float>(psi);
-    const float h = 0.5f * (1.0f + g);
-    thermo_m(a, Tp, c, h, q.dG, q.mu);
+    const float h = 0.5f * (1.0f + gg<float>(psi));
+    thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
     return q;
 }
 
-__device__ __forceinline__ double vertex4(double a0, double a1, double a2, double a3) {
-    return 0.25 * (((a0 + a1) + a2) + a3);
+__device__ __forceinline__ float quad4(float a0, float a1, float a2, float a3) {
+    return 0.25f * (((a0 + a1) + a2) + a3);
 }
 
 // FP32 anisotropic operator (fun.cu:1030-1139) with precomputed reciprocals
+template <bool IDENT>
 __device__ __forceinline__ float aniso_f32(const StepArgsMixed& A, float pmm, float p0m, float p1m,
                                            float pm0, float p00, float p10, float pm1, float p01,
                                            float p11, float& as, float& ak) {
@@ -167,21 +234,30 @@ __device__ __forceinline__ float aniso_f32(const StepArgsMixed& A, float pmm, fl
         ct = gx * ir;
         st = gy * ir;
     }
-    const float nx = fmaf(A.ops[0], ct, A.ops[1] * st);
-    const float ny = fmaf(A.ops[2], ct, A.ops[3] * st);
-    const float nz = fmaf(A.ops[4], ct, A.ops[5] * st);
-    const float ex = fmaf(-A.ops[0], st, A.ops[1] * ct);
-    const float ey = fmaf(-A.ops[2], st, A.ops[3] * ct);
-    const float ez = fmaf(-A.ops[4], st, A.ops[5] * ct);
-    const float nx2 = nx * nx, ny2 = ny * ny, nz2 = nz * nz;
-    const float nx3 = nx2 * nx, ny3 = ny2 * ny, nz3 = nz2 * nz;
-    const float np4 = fmaf(nx2, nx2, fmaf(ny2, ny2, nz2 * nz2));
-    const float d1 = 4.0f * fmaf(nx3, ex, fmaf(ny3, ey, nz3 * ez));
-    const float xe = nx * ex, ye = ny * ey, ze = nz * ez;
-    float d2 = fmaf(3.0f, xe * xe, -nx3 * nx);
-    d2 += fmaf(3.0f, ye * ye, -ny3 * ny);
-    d2 += fmaf(3.0f, ze * ze, -nz3 * nz);
-    d2 *= 4.0f;
+    float np4, d1, d2;
+    if (IDENT) {
+        // n' = (cos, sin, 0), dn'/dtheta = (-sin, cos, 0)
+        const float c2 = ct * ct, s2 = st * st;
+        np4 = fmaf(c2, c2, s2 * s2);
+        d1 = 4.0f * (ct * st) * (s2 - c2);
+        d2 = 4.0f * fmaf(6.0f * c2, s2, -np4);
+    } else {
+        const float nx = fmaf(A.ops[0], ct, A.ops[1] * st);
+        const float ny = fmaf(A.ops[2], ct, A.ops[3] * st);
+        const float nz = fmaf(A.ops[4], ct, A.ops[5] * st);
+        const float ex = fmaf(-A.ops[0], st, A.ops[1] * ct);
+        const float ey = fmaf(-A.ops[2], st, A.ops[3] * ct);
+        const float ez = fmaf(-A.ops[4], st, A.ops[5] * ct);
+        const float nx2 = nx * nx, ny2 = ny * ny, nz2 = nz * nz;
+        const float nx3 = nx2 * nx, ny3 = ny2 * ny, nz3 = nz2 * nz;
+        np4 = fmaf(nx2, nx2, fmaf(ny2, ny2, nz2 * nz2));
+        d1 = 4.0f * fmaf(nx3, ex, fmaf(ny3, ey, nz3 * ez));
+        const float xe = nx * ex, ye = ny * ey, ze = nz * ez;
+        d2 = fmaf(3.0f, xe * xe, -nx3 * nx);
+        d2 += fmaf(3.0f, ye * ye, -ny3 * ny);
+        d2 += fmaf(3.0f, ze * ze, -nz3 * nz);
+        d2 *= 4.0f;
+    }
     ak = fmaf(4.0f * A.epk1, np4, 1.0f - 3.0f * A.epk1);
     as = fmaf(4.0f * A.eps1, np4, 1.0f - 3.0f * A.eps1);
     const float da1 = 4.0f * A.eps1 * d1;
@@ -191,7 +267,31 @@ __device__ __forceinline__ float aniso_f32(const StepArgsMixed& A, float pmm, fl
                      fmaf(da2, as, da1 * da1) * fmaf(thy, gx, -thx * gy)));
 }
 
-__global__ __launch_bounds__(256) void step_mixed_kernel(StepArgsMixed a) {
+// one fine row of the 3-row window: the cell quantities and the values the
+// row needs from its i-1 / i+1 neighbours
+struct Slot {
+    CellM q;
+    float phL, phR, SR;
+};
+
+__device__ __forceinline__ void fill_slot(const StepArgsMixed& a, const double* lntab, int j,
+                                          float psi, double c, int ic, Slot& s) {
+    s.q = make_cell_m(a, lntab, row_t(a, j), psi, c, ic);
+    s.phL = shup(s.q.phi);
+    s.phR = shdn(s.q.phi);
+    s.SR = shdn(s.q.S);
+}
+
+// at least 4 wavefronts per SIMD (<= 128 VGPRs, no spills): the kernel is
+// latency-bound below that
+template <bool IDENT>
+__global__ __launch_bounds__(256, 4) void step_mixed_kernel(StepArgsMixed a) {
+    __shared__ double lntab[2 * RS_LN_TAB_N];
+    for (int q = threadIdx.y * blockDim.x + threadIdx.x; q < 2 * RS_LN_TAB_N;
+         q += blockDim.x * blockDim.y)
+        lntab[q] = kLnTab[q];
+    __syncthreads();
+
     const int lane = threadIdx.x;
     const int band = blockIdx.y * blockDim.y + threadIdx.y;
     if (band >= a.nbands) return;
@@ -201,72 +301,66 @@ __global__ __launch_bounds__(256) void step_mixed_kernel(StepArgsMixed a) {
     const int jb = 1 + band * a.band_h;
     const int je = min(jb + a.band_h - 1, a.Ny - 2);
     const int Nx = a.Nx;
-    const double K = a.kdc_dx2 * a.mu_den;
 
-    CellM m = make_cell_m(a, row_t(a, jb - 1), a.psi[(int64_t)(jb - 1) * Nx + ic],
-                          a.c[(int64_t)(jb - 1) * Nx + ic], ic);
-    CellM z = make_cell_m(a, row_t(a, jb), a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic],
-                          ic);
-    float phL_m = shup(m.phi), phR_m = shdn(m.phi);
-    float phL_z = shup(z.phi), phR_z = shdn(z.phi);
-    const double SR_m = shdn(m.S);
-    double SR_z = shdn(z.S);
+    Slot s0, s1, s2;
+    fill_slot(a, lntab, jb - 1, a.psi[(jb - 1) * Nx + ic], a.c[(jb - 1) * Nx + ic], ic, s0);
+    fill_slot(a, lntab, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, s1);
     // vertex (i+1/2, jb-1/2) and the y-face flux (i, jb-1/2)
-    double Vpm = vertex4(m.S, SR_m, z.S, SR_z);
-    double Vmm = shup(Vpm);
-    double Fy_prev = 0.25 * (((m.S + z.S) + Vpm) + Vmm) * (z.mu - m.mu);
+    float Vpm = quad4(s0.q.S, s0.SR, s1.q.S, s1.SR);
+    const float Vmm = shup(Vpm);
+    double Fy_prev = (double)quad4(s0.q.S, s1.q.S, Vpm, Vmm) * (s1.q.mu - s0.q.mu);
 
-    float nx_psi = a.psi[(int64_t)(jb + 1) * Nx + ic];
-    double nx_c = a.c[(int64_t)(jb + 1) * Nx + ic];
+    float nx_psi = a.psi[(jb + 1) * Nx + ic];
+    double nx_c = a.c[(jb + 1) * Nx + ic];
     float lh_acc = 0.0f;
 
-    for (int j = jb; j <= je; ++j) {
+    // output row j from rows M = j-1, Z = j and P = j+1 (filled here)
+    auto row = [&](int j, const Slot& M, const Slot& Z, Slot& P) {
         const float cur_psi = nx_psi;
         const double cur_c = nx_c;
         if (j + 2 <= je + 1) {
-            nx_psi = a.psi[(int64_t)(j + 2) * Nx + ic];
-            nx_c = a.c[(int64_t)(j + 2) * Nx + ic];
+            nx_psi = a.psi[(j + 2) * Nx + ic];
+            nx_c = a.c[(j + 2) * Nx + ic];
         }
-        const CellM p = make_cell_m(a, row_t(a, j + 1), cur_psi, cur_c, ic);
-        const float phL_p = shup(p.phi), phR_p = shdn(p.phi);
-        const double SR_p = shdn(p.S);
-        const double muR_z = shdn(z.mu);
+        fill_slot(a, lntab, j + 1, cur_psi, cur_c, ic, P);
+        const double muR_z = shdn(Z.q.mu);
         // vertex (i+1/2, j+1/2), its left neighbour, and the faces
-        const double Vpp = vertex4(z.S, SR_z, p.S, SR_p);
-        const double Vmp = shup(Vpp);
-        const double Fx = 0.25 * (((z.S + SR_z) + Vpp) + Vpm) * (muR_z - z.mu);  // (i+1/2, j)
-        const double Fxm = shup(Fx);                                              // (i-1/2, j)
-        const double Fy = 0.25 * (((z.S + p.S) + Vpp) + Vmp) * (p.mu - z.mu);    // (i, j+1/2)
+        const float Vpp = quad4(Z.q.S, Z.SR, P.q.S, P.SR);
+        const float Vmp = shup(Vpp);
+        const double Fx = (double)quad4(Z.q.S, Z.SR, Vpp, Vpm) * (muR_z - Z.q.mu);    // (i+1/2, j)
+        const double Fxm = shup(Fx);                                                  // (i-1/2, j)
+        const double Fy = (double)quad4(Z.q.S, P.q.S, Vpp, Vmp) * (P.q.mu - Z.q.mu);  // (i, j+1/2)
 
         if (own) {
             float as, ak;
-            const float an = aniso_f32(a, phL_m, m.phi, phR_m, phL_z, z.phi, phR_z, phL_p, p.phi,
-                                       phR_p, as, ak);
-            const float p00 = z.phi;
+            const float an = aniso_f32<IDENT>(a, M.phL, M.q.phi, M.phR, Z.phL, Z.q.phi, Z.phR, P.phL,
+                                              P.q.phi, P.phR, as, ak);
+            const float p00 = Z.q.phi;
             float term = fmaf(-(p00 * p00), p00, p00) + an;
-            term = fmaf(z.dG * dgg<float>(p00), a.dG_den_inv, term);
+            term = fmaf(Z.q.dG * dgg<float>(p00), a.dG_den_inv, term);
             float phn = fmaf(a.dt * ak * rcpf(as * as), term, p00);
             float psn = phn;
             if (psn > 1.0f) {
                 phn -= (psn - 1.0f);
                 psn = phn;
             }
-            const double dc = K * ((Fx - Fxm) + (Fy - Fy_prev));
-            const double cn = fmax(z.c + dc, 1.0e-6);
+            const double dc = a.K * ((Fx - Fxm) + (Fy - Fy_prev));
+            const double cn = fmax(Z.q.c + dc, 1.0e-6);
 
-            const int64_t row = (int64_t)j * Nx;
-            a.psi_n[row + i] = psn;
-            a.c_n[row + i] = cn;
+            float* prow = a.psi_n + j * Nx;
+            double* crow = a.c_n + j * Nx;
+            prow[i] = psn;
+            crow[i] = cn;
             if (i == 1) {
-                a.psi_n[row] = psn;
-                a.c_n[row] = cn;
+                prow[0] = psn;
+                crow[0] = cn;
             }
             if (i == Nx - 2) {
-                a.psi_n[row + Nx - 1] = psn;
-                a.c_n[row + Nx - 1] = cn;
+                prow[Nx - 1] = psn;
+                crow[Nx - 1] = cn;
             }
             if (j == a.Ny - 2 || j == 1) {
-                const int64_t g = (j == 1) ? (int64_t)(a.Ny - 1) * Nx : 0;
+                const int g = (j == 1) ? (a.Ny - 1) * Nx : 0;
                 a.psi_n[g + i] = psn;
                 a.c_n[g + i] = cn;
                 if (i == 1) {
@@ -279,32 +373,44 @@ __global__ __launch_bounds__(256) void step_mixed_kernel(StepArgsMixed a) {
                 }
             }
             if (a.lh_on && j <= a.lh_last_row) {
-                lh_acc += psn - z.psi;
+                lh_acc += psn - Z.q.psi;
                 if (j % 10 == 0) {
-                    a.lh[(int64_t)(j / 10 - 1) * Nx + i] = (double)lh_acc;
+                    a.lh[(j / 10 - 1) * Nx + i] = (double)lh_acc;
                     lh_acc = 0.0f;
                 }
             }
         }
-        m = z;
-        phL_m = phL_z;
-        phR_m = phR_z;
-        z = p;
-        phL_z = phL_p;
-        phR_z = phR_p;
-        SR_z = SR_p;
         Vpm = Vpp;
         Fy_prev = Fy;
+    };
+
+    // three rows per trip with the slot roles rotated: no register copies
+    // of the window between rows
+    for (int j = jb; j <= je; j += 3) {
+        row(j, s0, s1, s2);
+        if (j + 1 > je) break;
+        row(j + 1, s1, s2, s0);
+        if (j + 2 > je) break;
+        row(j + 2, s2, s0, s1);
     }
 }
 
-hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, hipStream_t s) {
+hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, hipStream_t s) {
     if (wpb < 1) wpb = 1;
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    hipLaunchKernelGGL(step_mixed_kernel, grid, block, 0, s, a);
+    if (ident)
+        hipLaunchKernelGGL(step_mixed_kernel<true>, grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL(step_mixed_kernel<false>, grid, block, 0, s, a);
     return hipGetLastError();
+}
+
+void sgte_table_info(double* T_lo, double* T_hi, double* max_err) {
+    *T_lo = kMixedTref + (double)RS_SGTE_T0;
+    *T_hi = kMixedTref + (double)RS_SGTE_T1;
+    *max_err = RS_SGTE_MAXERR;
 }
 
 // ---------------------------------------------------------------------------
@@ -318,25 +424,6 @@ __global__ void t32_copy_kernel(const double* T, float* T32, int64_t n, double T
 hipError_t launch_T32_copy(const double* T, float* T32, int64_t n, double Tref, hipStream_t s) {
     hipLaunchKernelGGL(t32_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, T, T32,
                        n, Tref);
-    return hipGetLastError();
-}
-
-__global__ void f32_to_f64_kernel(const float* a, double* b, int64_t n) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) b[q] = (double)a[q];
-}
-__global__ void f64_to_f32_kernel(const double* a, float* b, int64_t n) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) b[q] = (float)a[q];
-}
-hipError_t launch_f32_to_f64(const float* a, double* b, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(f32_to_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b,
-                       n);
-    return hipGetLastError();
-}
-hipError_t launch_f64_to_f32(const double* a, float* b, int64_t n, hipStream_t s) {
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, b,
-                       n);
     return hipGetLastError();
 }
 
@@ -425,112 +512,6 @@ hipError_t launch_tip_column_f32(const float* psi, int Nx, int Ny, float pin, in
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(max_int_f32_kernel, dim3(1), dim3(256), 0, s, rowbuf, Ny, out);
     return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
-// host: least-squares fit of the SGTE temperature dependence
-// ---------------------------------------------------------------------------
-namespace {
-
-void unary_funcs(double T, double f[4]) {
-    const Unaries u = sgte_unaries(T, log(T));
-    f[0] = u.ti_liq - u.nb_liq;
-    f[1] = u.ti_bcc - u.nb_bcc;
-    f[2] = u.ti_liq - u.ti_bcc;
-    f[3] = u.nb_liq - u.nb_bcc;
-}
-
-// fit degree kPolyDeg in t = (T - mid) / half on [a, b] (long double normal
-// equations on Chebyshev nodes)
-void fit_piece(double a, double b, double Tref, PolyTable* P, int piece) {
-    const int D = kPolyDeg;
-    const double mid = 0.5 * (a + b), half = 0.5 * (b - a);
-    P->cen[piece] = (float)(mid - Tref);
-    P->isc[piece] = half > 0 ? (float)(1.0 / half) : 0.0f;
-    const double hw = half > 0 ? 1.0 / (double)P->isc[piece] : 0.0;  // the scale the kernel uses
-    if (half <= 0) {
-        for (int f = 0; f < 4; ++f) {
-            P->c0[piece][f] = 0.0;
-            for (int k = 0; k < D; ++k) P->c[piece][f][k] = 0.0f;
-        }
-        return;
-    }
-    const int N = 64;
-    long double A[kPolyDeg + 1][kPolyDeg + 1] = {};
-    long double B[4][kPolyDeg + 1] = {};
-    for (int q = 0; q < N; ++q) {
-        const double tq = cos(M_PI * (q + 0.5) / N);
-        // the kernel forms t from T' and cen in FP32; fit against that t
-        const double T = mid + tq * hw;
-        double f[4];
-        unary_funcs(T, f);
-        long double tp[2 * kPolyDeg + 1];
-        tp[0] = 1.0L;
-        for (int k = 1; k <= 2 * D; ++k) tp[k] = tp[k - 1] * (long double)tq;
-        for (int r = 0; r <= D; ++r) {
-            for (int cc = 0; cc <= D; ++cc) A[r][cc] += tp[r + cc];
-            for (int g = 0; g < 4; ++g) B[g][r] += tp[r] * (long double)f[g];
-        }
-    }
-    // Gaussian elimination (SPD, no pivoting needed), four right-hand sides
-    for (int k = 0; k <= D; ++k) {
-        for (int r = k + 1; r <= D; ++r) {
-            const long double fct = A[r][k] / A[k][k];
-            for (int cc = k; cc <= D; ++cc) A[r][cc] -= fct * A[k][cc];
-            for (int g = 0; g < 4; ++g) B[g][r] -= fct * B[g][k];
-        }
-    }
-    for (int g = 0; g < 4; ++g) {
-        long double x[kPolyDeg + 1];
-        for (int r = D; r >= 0; --r) {
-            long double acc = B[g][r];
-            for (int cc = r + 1; cc <= D; ++cc) acc -= A[r][cc] * x[cc];
-            x[r] = acc / A[r][r];
-        }
-        P->c0[piece][g] = (double)x[0];
-        for (int k = 1; k <= D; ++k) P->c[piece][g][k - 1] = (float)x[k];
-    }
-}
-
-float eval_host(const PolyTable& P, int piece, int f, float Tp) {
-    const float t = (Tp - P.cen[piece]) * P.isc[piece];
-    float acc = P.c[piece][f][kPolyDeg - 1];
-    for (int k = kPolyDeg - 2; k >= 0; --k) acc = fmaf(acc, t, P.c[piece][f][k]);
-    return acc * t;
-}
-
-}  // namespace
-
-double fit_poly_table(const rs_params& p, double T_lo, double T_hi, double Tref, PolyTable* P) {
-    (void)p;
-    const double TB = 1941.0;  // G_Ti breakpoint (fun.cu:1629, 1639)
-    // stay inside the SGTE branches that do not change inside the window:
-    // G_Ti_Liq low branch starts at 1300 K, G_Nb changes at 2750 K
-    T_lo = std::max(T_lo, 1300.0);
-    T_hi = std::min(T_hi, 2749.0);
-    if (T_hi <= T_lo) T_hi = T_lo;
-    P->Tref = Tref;
-    P->lo = (float)(T_lo - Tref);
-    P->hi = (float)(T_hi - Tref);
-    const double a0 = T_lo, b0 = std::min(std::max(TB, T_lo), T_hi);
-    const double a1 = b0, b1 = T_hi;
-    P->tb = (float)(b0 - Tref);
-    fit_piece(a0, b0, Tref, P, 0);
-    fit_piece(a1, b1, Tref, P, 1);
-    // check on a dense grid exactly as the kernel evaluates (FP32 T')
-    double worst = 0.0;
-    for (int q = 0; q <= 4000; ++q) {
-        const float Tp = P->lo + (P->hi - P->lo) * (float)q / 4000.0f;
-        if (!(Tp >= P->lo && Tp < P->hi)) continue;
-        const int piece = Tp < P->tb ? 0 : 1;
-        double f[4];
-        unary_funcs(Tref + (double)Tp, f);
-        for (int g = 0; g < 4; ++g) {
-            const double v = P->c0[piece][g] + (double)eval_host(*P, piece, g, Tp);
-            worst = std::max(worst, fabs(v - f[g]));
-        }
-    }
-    return worst;
 }
 
 }  // namespace rs

@@ -1,12 +1,16 @@
 """Throughput benchmark of the coupled phi / c / T time-stepper.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--path fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--path mixed|fp64]
 
 One "step" is one pass of the reference loop body (fun.cu:448-509) over the
 whole N x N grid: fused phase-field/solute update, latent-heat block sums,
 coarse-grid heat equation and the pull-back check. Inputs are synthetic
 (the seeded planar Ti-10at%Nb front) and resident in HBM before timing.
-Prints ONE JSON line (rank 0).
+
+The metric compares the reduced-precision path with the plain FP64 path
+on the same 4096^2 workload, so both are timed in the same run: `value` is
+the headline path's MLUPS, `plain_fp64` holds the FP64 path's.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -24,19 +28,21 @@ for _p in (ROOT, PKG):
 
 METRIC = "MLUPS @4096² Ti-Nb: surrogate+FP32 vs plain FP64; tip-vel err ≤1%"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per cell update of the fused step kernel:
-# read + write of the persistent state psi and c (phi = max(psi,-1) is not
-# stored, fine T is rebuilt from the coarse grid)
-BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_fp64_kernel_4096.md")
+# algorithmic HBM bytes per cell update of the fused step kernel: read +
+# write of the persistent state (phi = max(psi,-1) is not stored, fine T is
+# rebuilt from the coarse grid)
+BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8,   # psi f64, c f64
+                  "mixed": 2 * 4 + 2 * 8}  # psi f32, c f64
+KERNEL = {"fp64": "step_fp64_kernel", "mixed": "step_mixed_kernel"}
+PROFILE = {"fp64": "step_fp64_kernel_4096.md", "mixed": "step_mixed_kernel_4096.md"}
 
 
-def _traffic_from_profile():
+def _traffic_from_profile(path: str):
     """HBM bytes per launch of the fused kernel as measured by rocprofv3 PMC
     (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
     recorded in profiles/; None when not measured."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(os.path.join(ROOT, "profiles", PROFILE[path])):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -68,15 +74,55 @@ def cpu_baseline(grid: int, seconds: float):
             "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s)"}
 
 
+def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool):
+    import rsolid
+    prec = rsolid.RS_PREC_MIXED if path == "mixed" else rsolid.RS_PREC_FP64
+    p = rsolid.config_params(grid, tfc=True, precision=prec)
+    ctx = rsolid.Context(p)
+    ctx.init_config(5)
+    ctx.step(warmup)
+    ctx.sync()
+    ctx.timing_enable(True)
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    k_ms, k_n = ctx.timing(0)
+    t_ms, _ = ctx.timing(1)
+    ctx.timing_enable(False)
+    kc = ctx.constants()
+    info = ctx.info()
+    copy_ms = ctx.calibrate_copy(grid * grid, 5) if calibrate else None
+    ctx.close()
+    cells = grid * grid
+    bpc = BYTES_PER_CELL[path]
+    achieved = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    return {
+        "mlups": cells * steps / el / 1e6,
+        "ms_per_step": el * 1e3 / steps,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": _traffic_from_profile(path), "kernel": KERNEL[path],
+                     "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_n,
+                     "bytes_per_cell": bpc},
+        "coarse_T_kernel_avg_ms": round(t_ms, 5),
+        "coarse_T": [kc.NxT, kc.NyT],
+        "info": info,
+        "copy_ms": copy_ms,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", type=int, default=4096)
-    ap.add_argument("--path", default="fp64", choices=["fp64"])
+    ap.add_argument("--path", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-reference", action="store_true",
+                    help="skip timing the plain FP64 path beside the headline path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
                          "(byte-counter calibration for rocprofv3 --pmc)")
@@ -86,57 +132,42 @@ def main() -> int:
               "(the y-band decomposition is a design note, not code).", file=sys.stderr)
         return 2
 
-    import rsolid
-    p = rsolid.config_params(a.grid, tfc=True)
-    ctx = rsolid.Context(p)
-    ctx.init_config(5)
-    ctx.step(a.warmup)
-    ctx.sync()
-    ctx.timing_enable(True)
-    t0 = time.perf_counter()
-    ctx.step(a.steps)
-    ctx.sync()
-    el = time.perf_counter() - t0
-    k_ms, k_n = ctx.timing(0)
-    t_ms, _ = ctx.timing(1)
-    ctx.timing_enable(False)
-    mass = ctx.solute_mass()
-    kc = ctx.constants()
-    copy_ms = ctx.calibrate_copy(a.grid * a.grid, 5) if a.calibrate_copy else None
-    ctx.close()
-
-    cells = a.grid * a.grid
-    mlups = cells * a.steps / el / 1e6
-    bpc = BYTES_PER_CELL[a.path]
-    achieved = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    head = run_path(a.path, a.grid, a.steps, a.warmup, a.calibrate_copy)
+    fp64 = None
+    if a.path != "fp64" and not a.no_fp64_reference:
+        fp64 = run_path("fp64", a.grid, max(1, a.steps // 2), a.warmup, False)
+    dtype = {"fp64": "f64", "mixed": "f32+f64"}[a.path]
+    desc = {"fp64": "plain FP64 path",
+            "mixed": "FP32 psi/phi and fine T, FP64 c / flux divergence / ln terms / "
+                     "time accumulation, exact CALPHAD (no surrogate)"}[a.path]
     out = {
         "metric": METRIC,
-        "value": round(mlups, 2),
+        "value": round(head["mlups"], 2),
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(el * 1e3 / a.steps, 5),
+        "ms_per_step": round(head["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic: seeded planar Ti-10at%Nb front (configuration preset), no dataset",
-        "config": {"workload": f"plain FP64 coupled phi/c/T step, {a.grid}x{a.grid} Ti-10at%Nb "
-                               "(coarse-grid heat equation, alpha = 100 D_l)",
-                   "grid": a.grid, "path": a.path, "coarse_T": [kc.NxT, kc.NyT]},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic_from_profile(),
-                     "kernel": "step_fp64_kernel", "kernel_avg_ms": round(k_ms, 5),
-                     "kernel_launches": k_n, "bytes_per_cell": bpc},
-        "coarse_T_kernel_avg_ms": round(t_ms, 5),
-        "solute_mass_end": mass,
+        "config": {"workload": f"coupled phi/c/T step, {a.grid}x{a.grid} Ti-10at%Nb, coarse-grid "
+                               f"heat equation (alpha = 100 D_l); {desc}",
+                   "grid": a.grid, "path": a.path, "coarse_T": head["coarse_T"]},
+        "roofline": head["roofline"],
+        "coarse_T_kernel_avg_ms": head["coarse_T_kernel_avg_ms"],
     }
-    if copy_ms is not None:
+    if fp64 is not None:
+        out["plain_fp64"] = {"value": round(fp64["mlups"], 2), "unit": "MLUPS",
+                             "ms_per_step": round(fp64["ms_per_step"], 5),
+                             "roofline": fp64["roofline"]}
+        out["speedup_vs_plain_fp64"] = round(head["mlups"] / fp64["mlups"], 3)
+    if head["copy_ms"] is not None:
         nb = 2 * 8 * a.grid * a.grid
-        out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(copy_ms, 5),
-                                   "GBps": round(nb / (copy_ms * 1e-3) / 1e9, 1)}
+        out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(head["copy_ms"], 5),
+                                   "GBps": round(nb / (head["copy_ms"] * 1e-3) / 1e9, 1)}
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(a.grid, a.cpu_seconds)
     print(json.dumps(out), flush=True)

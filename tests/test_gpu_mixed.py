@@ -1,0 +1,114 @@
+"""Mixed-precision path (psi/phi and fine T in FP32; c, flux divergence,
+ln terms and time accumulation in FP64) against the FP64 CPU oracle.
+
+Stated tolerances (after 2000 steps of the 256^2 configuration):
+  psi   L_inf <= 2e-6 absolute  (FP32 storage: ulp(1) = 1.2e-7 per step)
+  c     L_inf <= 1e-7 relative
+  T     L_inf <= 1e-8 relative  (coarse T advanced in FP64)
+  solute mass trajectory within 1e-10 of the oracle's (relative)
+  tip x within 1e-4 cells
+and exact interior conservation (<= 1e-13) with uniform T.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-mixed]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("tfc", [False, True])
+def test_mixed_tracks_fp64_oracle(rs, oracle_mod, tfc):
+    pm = rs.config_params(256, tfc, precision=RS_PREC_MIXED)
+    pd = rs.config_params(256, tfc)
+    say("mixed vs oracle, 256^2, tfc", tfc)
+    cm = rs.Context(pm)
+    cm.init_config(7)
+    info = cm.info()
+    say("poly table", info)
+    assert info["poly_err"] < 1e-3
+    st = oracle_mod.State(pd)
+    st.init_config(7)
+    m0, mo0 = cm.solute_mass(), st.solute_mass()
+    for n in (1, 199, 1800):
+        cm.step(n)
+        st.step(n)
+        epsi = float(np.max(np.abs(cm.download(RS_FIELD_PSI) - st.get(RS_FIELD_PSI))))
+        ec = _rel(cm.download(RS_FIELD_C), st.get(RS_FIELD_C))
+        dm = (cm.solute_mass() - m0) / m0 - (st.solute_mass() - mo0) / mo0
+        say("step", cm.constants().step, "psi Linf", epsi, "c rel", ec, "mass-drift diff", dm)
+        assert epsi <= 2e-6
+        assert ec <= 1e-7
+        assert abs(dm) <= 1e-10
+        if tfc:
+            eT = _rel(cm.download(RS_FIELD_T), st.get(RS_FIELD_T))
+            assert eT <= 1e-8
+    tm, to = cm.tip(), st.tip()
+    assert tm["i"] == to["i"] and tm["j"] == to["j"]
+    assert tm["x"] == pytest.approx(to["x"], abs=1e-4)
+    cm.close()
+
+
+def test_mixed_conserves_solute_exactly(rs):
+    p = rs.config_params(256, False, precision=RS_PREC_MIXED)
+    p.GG = 0.0
+    p.q_solid = 0.0
+    say("mixed conservation, uniform T")
+    ctx = rs.Context(p)
+    ctx.init_config(3)
+    m0 = ctx.solute_mass()
+    ctx.step(2000)
+    c = ctx.download(RS_FIELD_C)
+    assert c.min() > 1e-6
+    drift = (ctx.solute_mass() - m0) / m0
+    say("drift", drift)
+    assert abs(drift) < 1e-13
+    ctx.close()
+
+
+def test_mixed_exact_fallback_outside_table(rs, oracle_mod):
+    """A frozen-T run whose gradient spans more than the polynomial window:
+    cells outside it take the exact FP64 routine; results still track the
+    oracle."""
+    pm = rs.config_params(256, False, precision=RS_PREC_MIXED)
+    pm.GG = 2.0          # 256 cells * 0.2 nm * 2 K/nm = 102 K span
+    pd = pm.copy()
+    pd.precision = 0
+    say("mixed exact fallback")
+    cm = rs.Context(pm)
+    cm.init_config(9)
+    info = cm.info()
+    st = oracle_mod.State(pd)
+    st.init_config(9)
+    cm.step(300)
+    st.step(300)
+    assert float(np.max(np.abs(cm.download(RS_FIELD_PSI) - st.get(RS_FIELD_PSI)))) <= 2e-6
+    assert _rel(cm.download(RS_FIELD_C), st.get(RS_FIELD_C)) <= 1e-7
+    say("window", info["T_lo"], info["T_hi"])
+    cm.close()
+
+
+def test_mixed_full_size_determinism(rs):
+    p = rs.config_params(4096, True, precision=RS_PREC_MIXED)
+    say("mixed 4096^2 determinism")
+    outs = []
+    for _ in range(2):
+        ctx = rs.Context(p)
+        ctx.init_config(5)
+        ctx.step(100)
+        outs.append((ctx.download(RS_FIELD_PSI), ctx.download(RS_FIELD_C)))
+        ctx.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.isfinite(outs[0][0]).all() and np.isfinite(outs[0][1]).all()
