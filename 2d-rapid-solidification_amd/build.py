@@ -36,8 +36,10 @@ PER_FILE = {
     # FMAs in the mixed stepper are written out explicitly; no implicit
     # contraction, so duplicated face-flux evaluations round identically
     "rs_mixed.hip": ["-ffp-contract=off"],
+    "rs_surr.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_params.hip", "rs_tip.hip", "rs_capi.hip"]
+SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
+           "rs_capi.hip", "rs_surr.hip"]
 
 LIB = os.path.join(PKG, "rsolid", "librsolid.so")
 ORACLE_SRC = os.path.join(ROOT, "oracle", "rs_oracle.c")

@@ -41,11 +41,14 @@ int check_params(const rs_params& p, std::string* why) {
     if (p.Nx < 8 || p.Ny < 8) { *why = "grid must be at least 8x8"; return RS_ERR_ARG; }
     if (p.S < 1 || p.dx <= 0 || p.W0 <= 0) { *why = "S, dx and W0 must be positive"; return RS_ERR_ARG; }
     if (p.ratio_dxT_dx != 10) { *why = "ratio_dxT_dx must be 10 (10-row latent-heat blocks)"; return RS_ERR_ARG; }
-    if (p.precision != RS_PREC_FP64 && p.precision != RS_PREC_MIXED) { *why = "bad precision"; return RS_ERR_ARG; }
+    if (p.precision != RS_PREC_FP64 && p.precision != RS_PREC_MIXED && p.precision != RS_PREC_SURROGATE) {
+        *why = "bad precision";
+        return RS_ERR_ARG;
+    }
     if (p.ifTFC && p.Ny < 22) { *why = "coupled T needs Ny >= 22"; return RS_ERR_ARG; }
     if (p.i_tip_target < 1 || p.i_tip_target > p.Nx - 2) { *why = "i_tip_target outside the grid"; return RS_ERR_ARG; }
     if ((int64_t)p.Nx * p.Ny >= (int64_t)1 << 31) { *why = "grid exceeds 2^31 cells (32-bit offsets)"; return RS_ERR_ARG; }
-    if (p.precision == RS_PREC_MIXED) {
+    if (p.precision != RS_PREC_FP64) {
         // the mixed stepper indexes the coarse-T rows with integer arithmetic
         const double dxT = p.dx * p.ratio_dxT_dx;
         if (fabs(dxT - (double)lround(dxT)) > 1e-12) { *why = "mixed path needs dx*ratio_dxT_dx integral"; return RS_ERR_ARG; }
@@ -76,6 +79,7 @@ void free_all(rs_ctx* ctx) {
     float* fs[] = {ctx->psi32[0], ctx->psi32[1], ctx->T32[0], ctx->T32[1]};
     for (float* f : fs)
         if (f) (void)hipFree(f);
+    surr_free(ctx);
     if (ctx->rowi) (void)hipFree(ctx->rowi);
     if (ctx->outi) (void)hipFree(ctx->outi);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
@@ -182,7 +186,7 @@ int refresh_T32(rs_ctx* ctx) {
 StepArgsMixed step_args_mixed(rs_ctx* ctx) {
     const rs_params& p = ctx->p;
     const rs_constants& k = ctx->k;
-    StepArgsMixed a;
+    StepArgsMixed a{};
     a.psi = ctx->psi32[ctx->cur];
     a.c = ctx->c[ctx->cur];
     a.psi_n = ctx->psi32[ctx->cur ^ 1];
@@ -311,7 +315,8 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     // row jT+1 reaches row NyT_all, with weight 0, when NyT % 4 == 0)
     const size_t tb = sizeof(double) * (size_t)(ctx->nT + ctx->k.NxT_all);
     const size_t lhb = sizeof(double) * (size_t)p->Nx * (size_t)(ctx->k.NyT > 2 ? ctx->k.NyT - 2 : 1);
-    ctx->mixed = p->precision == RS_PREC_MIXED;
+    ctx->mixed = p->precision != RS_PREC_FP64;
+    ctx->surrogate = p->precision == RS_PREC_SURROGATE;
     const size_t fb32 = sizeof(float) * (size_t)ctx->n;
     const size_t tb32 = sizeof(float) * (size_t)(ctx->nT + ctx->k.NxT_all);
     bool ok = true;
@@ -529,11 +534,16 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             if (rc) return rc;
         }
         if (ctx->mixed) {
-            const StepArgsMixed a = step_args_mixed(ctx);
+            StepArgsMixed a = step_args_mixed(ctx);
+            if (ctx->surrogate) {
+                if (!surr_ready(ctx))
+                    return fail(ctx, RS_ERR_STATE, "surrogate path needs trained weights");
+                surr_step_args(ctx, a);
+            }
             const double* o = ctx->orient;
             const bool ident = o[0] == 1.0 && o[1] == 0.0 && o[3] == 0.0 && o[4] == 1.0 &&
                                o[6] == 0.0 && o[7] == 0.0;
-            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->stream));
+            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->surrogate, ctx->stream));
         } else {
             const StepArgsF64 a = step_args(ctx);
             HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));

@@ -23,6 +23,12 @@ struct TipFit {
 };
 int precise_tip(const rs_params& p, const double* psi, TipFit* out);
 
+// learned CALPHAD surrogate (rs_surr.hip)
+struct SurrState;
+void surr_free(rs_ctx* ctx);
+bool surr_ready(const rs_ctx* ctx);
+void surr_step_args(rs_ctx* ctx, StepArgsMixed& a);
+
 }  // namespace rs
 
 struct rs_ctx {
@@ -40,6 +46,8 @@ struct rs_ctx {
     float* psi32[2] = {nullptr, nullptr};
     float* T32[2] = {nullptr, nullptr};
     bool mixed = false;
+    bool surrogate = false;  // RS_PREC_SURROGATE: surrogate thermodynamics in the step
+    rs::SurrState* surr = nullptr;
     double* lh = nullptr;
     double* rowd = nullptr;
     int* rowi = nullptr;

@@ -189,13 +189,71 @@ __device__ __forceinline__ void thermo_m(const StepArgsMixed& a, const double* l
     }
 }
 
+// learned-surrogate thermodynamics for the 64 cells of a wavefront row
+// (called with every lane active: the MFMA forward pass is wavefront-wide).
+// Cells outside the training domain or in a bin the trust table rejects
+// take the exact routine; nfb counts them (wavefront-uniform). So do the two
+// columns at each x edge and the two rows at each y edge: a ghost copies c
+// and psi but its T is interpolated at its own position (fun.cu:1220-1230,
+// 1585-1606), so the edge faces carry a flux driven only by that T
+// difference, which the network's rounding noise would swamp and turn into
+// a solute-mass drift. Lanes past the last column (duplicates of it) keep
+// the network value, which nothing reads, and are not counted.
+struct SurrLds {
+    const MlpInfer* F;
+    const uint64_t* trust;
+};
+
+template <bool SPLIT>
+__device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* lntab,
+                                         const SurrLds& sl, float Tp, double c, float h, int i,
+                                         int j, float& dG, double& mu, unsigned& nfb,
+                                         unsigned& nall) {
+    const int lane = threadIdx.x;  // blockDim.x == 64
+    const float cf = (float)c;
+    const float ain = fmaf(Tp, a.s_a_scale, a.s_a_off);
+    const float bin = fmaf(-5.0f, cf, 4.0f);  // (x_Nb - 0.2)/0.2, x_Nb = 1 - c
+    // an opaque zero offset keeps the fragment reads inside the row: hoisted
+    // out of the row loop they would pin ~160 registers
+    int zoff = 0;
+    asm volatile("" : "+v"(zoff));
+    const MlpInfer& F =
+        *reinterpret_cast<const MlpInfer*>(reinterpret_cast<const char*>(sl.F) + zoff);
+    const mlp::f4 y = mlp::forward<SPLIT>(F, mlp::input_frag(ain, bin), lane);
+    const bool real = (int)(blockIdx.x * kTileOut + lane) < a.Nx;
+    bool ok = fabsf(ain) <= 1.0f && fabsf(bin) <= 1.0f && i > 1 && i < a.Nx - 2 && j > 1 &&
+              j < a.Ny - 2;
+    if (ok) {
+        const int ia = min((int)((ain + 1.0f) * (0.5f * kTrustN)), kTrustN - 1);
+        const int ib = min((int)((bin + 1.0f) * (0.5f * kTrustN)), kTrustN - 1);
+        ok = (sl.trust[ib] >> ia) & 1ULL;
+    }
+    ok = ok || !real;
+    nfb += (unsigned)__popcll(__ballot(!ok));
+    nall += (unsigned)__popcll(__ballot(real));
+    if (ok) {
+        const double onemc = 1.0 - c;
+        double lr = 0.0;
+        if (c > 1.e-8 && onemc > 1.e-8) lr = ln_tab(ratio_c(c, onemc), lntab);
+        dG = fmaf(y[0], a.y_std[0], a.y_mean[0]);
+        const float muL = fmaf(y[2], a.y_std[2], a.y_mean[2]);
+        const float dmu = fmaf(y[3], a.y_std[3], a.y_mean[3]);
+        mu = (double)fmaf(h, dmu, muL) + a.RR * (kMixedTref + (double)Tp) * lr;
+    } else {
+        thermo_m(a, lntab, Tp, c, h, dG, mu);
+    }
+}
+
 struct CellM {
     float psi, phi, dG, S;
     double c, mu;
 };
 
+template <int SURR>
 __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const double* lntab,
-                                             const RowT& rt, float psi, double c, int i) {
+                                             const SurrLds& sl, const RowT& rt, float psi,
+                                             double c, int i, int j, unsigned& nfb,
+                                             unsigned& nall) {
     CellM q;
     q.psi = psi;
     q.c = c;
@@ -204,7 +262,10 @@ __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const doubl
     q.S = qq<float>(a.AA, a.q_solid, psi) * (cf * (1.0f - cf));
     const float Tp = temp_m(a, rt, i);
     const float h = 0.5f * (1.0f + gg<float>(psi));
-    thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
+    if constexpr (SURR != 0)
+        thermo_s<SURR == 2>(a, lntab, sl, Tp, c, h, i, j, q.dG, q.mu, nfb, nall);
+    else
+        thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
     return q;
 }
 
@@ -274,23 +335,39 @@ struct Slot {
     float phL, phR, SR;
 };
 
-__device__ __forceinline__ void fill_slot(const StepArgsMixed& a, const double* lntab, int j,
-                                          float psi, double c, int ic, Slot& s) {
-    s.q = make_cell_m(a, lntab, row_t(a, j), psi, c, ic);
+template <int SURR>
+__device__ __forceinline__ void fill_slot(const StepArgsMixed& a, const double* lntab,
+                                          const SurrLds& sl, int j, float psi, double c, int ic,
+                                          Slot& s, unsigned& nfb, unsigned& nall) {
+    s.q = make_cell_m<SURR>(a, lntab, sl, row_t(a, j), psi, c, ic, j, nfb, nall);
     s.phL = shup(s.q.phi);
     s.phR = shdn(s.q.phi);
     s.SR = shdn(s.q.S);
 }
 
-// at least 4 wavefronts per SIMD (<= 128 VGPRs, no spills): the kernel is
-// latency-bound below that
-template <bool IDENT>
-__global__ __launch_bounds__(256, 4) void step_mixed_kernel(StepArgsMixed a) {
+// SURR: 0 exact thermodynamics, 1 surrogate, 2 surrogate with split
+// activations. At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
+// the exact thermodynamics: the kernel is latency-bound below that. The
+// surrogate variants hold the MFMA operands and run at 2.
+template <bool IDENT, int SURR>
+__global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
     __shared__ double lntab[2 * RS_LN_TAB_N];
-    for (int q = threadIdx.y * blockDim.x + threadIdx.x; q < 2 * RS_LN_TAB_N;
-         q += blockDim.x * blockDim.y)
-        lntab[q] = kLnTab[q];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nthr = blockDim.x * blockDim.y;
+    for (int q = tid; q < 2 * RS_LN_TAB_N; q += nthr) lntab[q] = kLnTab[q];
+    SurrLds sl{nullptr, nullptr};
+    if constexpr (SURR != 0) {
+        __shared__ MlpInfer smlp;
+        __shared__ uint64_t strust[kTrustN];
+        const uint4* src = reinterpret_cast<const uint4*>(a.mlp);
+        uint4* dst = reinterpret_cast<uint4*>(&smlp);
+        for (int q = tid; q < (int)(sizeof(MlpInfer) / 16); q += nthr) dst[q] = src[q];
+        for (int q = tid; q < kTrustN; q += nthr) strust[q] = a.trust[q];
+        sl.F = &smlp;
+        sl.trust = strust;
+    }
     __syncthreads();
+    unsigned nfb = 0, nall = 0;
 
     const int lane = threadIdx.x;
     const int band = blockIdx.y * blockDim.y + threadIdx.y;
@@ -303,8 +380,9 @@ __global__ __launch_bounds__(256, 4) void step_mixed_kernel(StepArgsMixed a) {
     const int Nx = a.Nx;
 
     Slot s0, s1, s2;
-    fill_slot(a, lntab, jb - 1, a.psi[(jb - 1) * Nx + ic], a.c[(jb - 1) * Nx + ic], ic, s0);
-    fill_slot(a, lntab, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, s1);
+    fill_slot<SURR>(a, lntab, sl, jb - 1, a.psi[(jb - 1) * Nx + ic], a.c[(jb - 1) * Nx + ic], ic,
+                    s0, nfb, nall);
+    fill_slot<SURR>(a, lntab, sl, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, s1, nfb, nall);
     // vertex (i+1/2, jb-1/2) and the y-face flux (i, jb-1/2)
     float Vpm = quad4(s0.q.S, s0.SR, s1.q.S, s1.SR);
     const float Vmm = shup(Vpm);
@@ -322,7 +400,7 @@ __global__ __launch_bounds__(256, 4) void step_mixed_kernel(StepArgsMixed a) {
             nx_psi = a.psi[(j + 2) * Nx + ic];
             nx_c = a.c[(j + 2) * Nx + ic];
         }
-        fill_slot(a, lntab, j + 1, cur_psi, cur_c, ic, P);
+        fill_slot<SURR>(a, lntab, sl, j + 1, cur_psi, cur_c, ic, P, nfb, nall);
         const double muR_z = shdn(Z.q.mu);
         // vertex (i+1/2, j+1/2), its left neighbour, and the faces
         const float Vpp = quad4(Z.q.S, Z.SR, P.q.S, P.SR);
@@ -393,17 +471,35 @@ __global__ __launch_bounds__(256, 4) void step_mixed_kernel(StepArgsMixed a) {
         if (j + 2 > je) break;
         row(j + 2, s2, s0, s1);
     }
+    if constexpr (SURR != 0) {
+        if (lane == 0) {
+            atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
+            atomicAdd(&a.fb_count[1], (unsigned long long)nall);
+        }
+    }
 }
 
-hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, hipStream_t s) {
+hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s) {
     if (wpb < 1) wpb = 1;
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    if (ident)
-        hipLaunchKernelGGL(step_mixed_kernel<true>, grid, block, 0, s, a);
-    else
-        hipLaunchKernelGGL(step_mixed_kernel<false>, grid, block, 0, s, a);
+    if (surr && a.s_split) {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 2>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 2>), grid, block, 0, s, a);
+    } else if (surr) {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 1>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 1>), grid, block, 0, s, a);
+    } else {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 0>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 0>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

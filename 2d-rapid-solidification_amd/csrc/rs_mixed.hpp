@@ -4,6 +4,7 @@
 #pragma once
 
 #include "rs_kernels.hpp"
+#include "rs_mlp.hpp"
 
 namespace rs {
 
@@ -31,12 +32,21 @@ struct StepArgsMixed {
     double L0l, L1l, RR;
     float dL0, dL1;          // L0_liq - L0_bcc, L1_liq - L1_bcc
     RK rk;                   // exact routine for cells outside the table
+    // learned surrogate (surrogate kernel only): BF16 fragments, trust
+    // bits, fallback counters [fallback, all], input/output normalisation
+    const MlpInfer* mlp;
+    const uint64_t* trust;
+    unsigned long long* fb_count;
+    float s_a_scale, s_a_off;  // a = T' s_a_scale + s_a_off
+    float y_mean[4], y_std[4];
+    int s_split;               // split (hi + lo) BF16 activations
 };
 
 // ident: the crystal orientation is the identity (the reference's only
 // built orientation, alpha0 = beta0 = gamma0 = 0), which removes the z row
 // of the rotated normal
-hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, hipStream_t s);
+// surr: thermodynamics from the learned surrogate with exact fallback
+hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s);
 hipError_t launch_T32_copy(const double* T, float* T32, int64_t n, double Tref, hipStream_t s);
 hipError_t launch_init_fields_f32(float* psi, double* c, const double* x0, int Nx, int Ny,
                                   double dx, int mode, double cinf, double c_liq, double c_sol,

@@ -15,7 +15,8 @@ import numpy as np
 from .abi import (  # noqa: F401  (re-exported)
     EXPORTED, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_STATE, RS_FIELD_C,
     RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE, RS_OK, RS_PREC_FP64, RS_PREC_MIXED,
-    RsConstants, RsParams, RsTipRow,
+    RS_PREC_SURROGATE, RS_SURR_NMETA, RS_SURR_NPARAM, RS_SURR_NREPORT, RsConstants, RsParams,
+    RsSurrogateOpts, RsTipRow,
 )
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librsolid.so")
@@ -60,6 +61,14 @@ def _load() -> C.CDLL:
         "rs_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "rs_calibrate_copy": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double)]),
         "rs_context_info": (C.c_int, [vp, P(C.c_double), C.c_int32]),
+        "rs_surrogate_defaults": (C.c_int, [P(RsSurrogateOpts)]),
+        "rs_surrogate_train": (C.c_int, [vp, P(RsSurrogateOpts), P(C.c_double), C.c_int32]),
+        "rs_surrogate_get": (C.c_int, [vp, P(C.c_float), C.c_int32, P(C.c_double), C.c_int32]),
+        "rs_surrogate_set": (C.c_int, [vp, P(C.c_float), C.c_int32, P(C.c_double), C.c_int32]),
+        "rs_surrogate_grad": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float),
+                                        P(C.c_double)]),
+        "rs_surrogate_eval": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_double)]),
+        "rs_surrogate_stats": (C.c_int, [vp, P(C.c_double), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -109,6 +118,25 @@ def device_name() -> str:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def surrogate_defaults(**kw) -> RsSurrogateOpts:
+    """Default training options of the CALPHAD surrogate, overridden by kw."""
+    o = RsSurrogateOpts()
+    rc = lib.rs_surrogate_defaults(C.byref(o))
+    if rc:
+        raise RsError(rc, "rs_surrogate_defaults")
+    for key, val in kw.items():
+        setattr(o, key, val)
+    return o
+
+
+SURR_REPORT_KEYS = ("mse", "trusted_frac", "err_dG", "err_fS", "err_muL", "err_dmu",
+                    "max_res_trusted", "train_s", "steps", "T_lo", "T_hi", "mse0")
 
 
 class Context:
@@ -217,6 +245,50 @@ class Context:
         ms = C.c_double()
         self._chk(lib.rs_calibrate_copy(self._h, n, reps, C.byref(ms)), "rs_calibrate_copy")
         return ms.value
+
+    # -- learned CALPHAD surrogate ----------------------------------------
+    def surrogate_train(self, opts: RsSurrogateOpts | None = None, **kw) -> dict:
+        o = opts if opts is not None else surrogate_defaults(**kw)
+        rep = (C.c_double * RS_SURR_NREPORT)()
+        self._chk(lib.rs_surrogate_train(self._h, C.byref(o), rep, RS_SURR_NREPORT),
+                  "rs_surrogate_train")
+        return dict(zip(SURR_REPORT_KEYS, list(rep)))
+
+    def surrogate_get(self) -> tuple[np.ndarray, np.ndarray]:
+        P = np.empty(RS_SURR_NPARAM, dtype=np.float32)
+        meta = np.empty(RS_SURR_NMETA, dtype=np.float64)
+        self._chk(lib.rs_surrogate_get(self._h, _fptr(P), P.size, _dptr(meta), meta.size),
+                  "rs_surrogate_get")
+        return P, meta
+
+    def surrogate_set(self, params: np.ndarray, meta: np.ndarray) -> None:
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        m = np.ascontiguousarray(meta, dtype=np.float64)
+        self._chk(lib.rs_surrogate_set(self._h, _fptr(P), P.size, _dptr(m), m.size),
+                  "rs_surrogate_set")
+
+    def surrogate_grad(self, X: np.ndarray, Yn: np.ndarray) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(Yn, dtype=np.float32)
+        n = x.shape[0]
+        g = np.empty(RS_SURR_NPARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(lib.rs_surrogate_grad(self._h, _fptr(x), _fptr(y), n, _fptr(g), C.byref(loss)),
+                  "rs_surrogate_grad")
+        return g, loss.value
+
+    def surrogate_eval(self, T: np.ndarray, c: np.ndarray) -> np.ndarray:
+        """(f_L, f_S, mu_L, mu_S, trusted) per (T, c) point, J/mol."""
+        tc = np.ascontiguousarray(np.stack([np.ravel(T), np.ravel(c)], axis=1), dtype=np.float64)
+        out = np.empty((tc.shape[0], 5), dtype=np.float64)
+        self._chk(lib.rs_surrogate_eval(self._h, _dptr(tc), tc.shape[0], _dptr(out)),
+                  "rs_surrogate_eval")
+        return out
+
+    def surrogate_stats(self, reset: bool = False) -> dict:
+        v = (C.c_double * 3)()
+        self._chk(lib.rs_surrogate_stats(self._h, v, 1 if reset else 0), "rs_surrogate_stats")
+        return {"fallback": v[0], "evaluated": v[1], "fallback_frac": v[2]}
 
     def timing(self, which: int) -> tuple[float, int]:
         ms = C.c_double()

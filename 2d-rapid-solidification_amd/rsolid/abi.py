@@ -16,6 +16,11 @@ RS_ERR_STATE = -5
 
 RS_PREC_FP64 = 0
 RS_PREC_MIXED = 1
+RS_PREC_SURROGATE = 2
+
+RS_SURR_NPARAM = 4612
+RS_SURR_NMETA = 14
+RS_SURR_NREPORT = 12
 
 RS_FIELD_C = 0
 RS_FIELD_PSI = 1
@@ -79,11 +84,24 @@ class RsTipRow(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class RsSurrogateOpts(C.Structure):
+    _fields_ = [
+        ("n_uniform", C.c_int64), ("n_near", C.c_int64), ("seed", C.c_uint64),
+        ("epochs", _I), ("batch", _I), ("T_lo", _D), ("T_hi", _D), ("lr", _D),
+        ("lr_final", _D), ("residual_tol", _D), ("n_curve", _I), ("split_act", _I),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 # every extern "C" symbol include/rsolid.h declares
 EXPORTED = [
     "rs_abi_info", "rs_params_reference", "rs_params_config", "rs_derive", "rs_create", "rs_destroy",
     "rs_last_error", "rs_device_name", "rs_init_reference", "rs_init_config", "rs_upload",
     "rs_download", "rs_get_constants", "rs_set_constants", "rs_step", "rs_tip_column",
     "rs_pull_back", "rs_solute_mass", "rs_tip", "rs_sync", "rs_timing_enable", "rs_timing",
-    "rs_calibrate_copy", "rs_context_info",
+    "rs_calibrate_copy", "rs_context_info", "rs_surrogate_defaults", "rs_surrogate_train",
+    "rs_surrogate_get", "rs_surrogate_set", "rs_surrogate_grad", "rs_surrogate_eval",
+    "rs_surrogate_stats",
 ]

@@ -1,15 +1,20 @@
 """Throughput benchmark of the coupled phi / c / T time-stepper.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N] [--path mixed|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid N]
+                    [--path surrogate|mixed|fp64]
 
 One "step" is one pass of the reference loop body (fun.cu:448-509) over the
 whole N x N grid: fused phase-field/solute update, latent-heat block sums,
 coarse-grid heat equation and the pull-back check. Inputs are synthetic
 (the seeded planar Ti-10at%Nb front) and resident in HBM before timing.
 
-The metric compares the reduced-precision path with the plain FP64 path
-on the same 4096^2 workload, so both are timed in the same run: `value` is
-the headline path's MLUPS, `plain_fp64` holds the FP64 path's.
+The metric compares the surrogate + FP32 path with the plain FP64 path on
+the same 4096^2 workload, so the three paths (surrogate + FP32 fields, FP32
+fields with the exact CALPHAD table, plain FP64) are timed in the same run
+from the same initial state: `value` is the headline path's MLUPS, `paths`
+holds all three, and `tip_velocity_err` compares each path's tip advance over
+the timed steps with the plain FP64 path's. The surrogate is trained on the
+GPU before the warm-up (outside the timed region; its time is reported).
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -28,13 +33,20 @@ for _p in (ROOT, PKG):
 
 METRIC = "MLUPS @4096² Ti-Nb: surrogate+FP32 vs plain FP64; tip-vel err ≤1%"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 # algorithmic HBM bytes per cell update of the fused step kernel: read +
 # write of the persistent state (phi = max(psi,-1) is not stored, fine T is
 # rebuilt from the coarse grid)
-BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8,   # psi f64, c f64
-                  "mixed": 2 * 4 + 2 * 8}  # psi f32, c f64
-KERNEL = {"fp64": "step_fp64_kernel", "mixed": "step_mixed_kernel"}
-PROFILE = {"fp64": "step_fp64_kernel_4096.md", "mixed": "step_mixed_kernel_4096.md"}
+BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8,        # psi f64, c f64
+                  "mixed": 2 * 4 + 2 * 8,       # psi f32, c f64
+                  "surrogate": 2 * 4 + 2 * 8}
+# algorithmic flops of one surrogate evaluation (2->64->64->4 MLP, one per
+# cell): 2 (2*64 + 64*64 + 64*4)
+FLOPS_PER_CELL_MLP = 2 * (2 * 64 + 64 * 64 + 64 * 4)
+KERNEL = {"fp64": "step_fp64_kernel", "mixed": "step_mixed_kernel<IDENT=1,SURR=0>",
+          "surrogate": "step_mixed_kernel<IDENT=1,SURR=2>"}
+PROFILE = {"fp64": "step_fp64_kernel_4096.md", "mixed": "step_mixed_kernel_4096.md",
+           "surrogate": "step_mixed_kernel_surrogate_4096.md"}
 
 
 def _traffic_from_profile(path: str):
@@ -74,14 +86,23 @@ def cpu_baseline(grid: int, seconds: float):
             "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s)"}
 
 
-def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool):
+def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
+             train_epochs: int | None = None):
     import rsolid
-    prec = rsolid.RS_PREC_MIXED if path == "mixed" else rsolid.RS_PREC_FP64
+    prec = {"fp64": rsolid.RS_PREC_FP64, "mixed": rsolid.RS_PREC_MIXED,
+            "surrogate": rsolid.RS_PREC_SURROGATE}[path]
     p = rsolid.config_params(grid, tfc=True, precision=prec)
     ctx = rsolid.Context(p)
     ctx.init_config(5)
+    train = None
+    if path == "surrogate":
+        train = (ctx.surrogate_train() if train_epochs is None
+                 else ctx.surrogate_train(epochs=train_epochs))
     ctx.step(warmup)
     ctx.sync()
+    x0 = _tip_x(ctx)
+    if path == "surrogate":
+        ctx.surrogate_stats(reset=True)
     ctx.timing_enable(True)
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -90,26 +111,45 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool):
     k_ms, k_n = ctx.timing(0)
     t_ms, _ = ctx.timing(1)
     ctx.timing_enable(False)
+    fb = ctx.surrogate_stats() if path == "surrogate" else None
+    x1 = _tip_x(ctx)
     kc = ctx.constants()
     info = ctx.info()
     copy_ms = ctx.calibrate_copy(grid * grid, 5) if calibrate else None
     ctx.close()
     cells = grid * grid
     bpc = BYTES_PER_CELL[path]
-    achieved = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    gbs = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(path)}
+    if path == "surrogate":
+        tf = FLOPS_PER_CELL_MLP * cells / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
+        roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
+                "traffic": hbm["traffic"], "flops_per_cell": FLOPS_PER_CELL_MLP, "hbm": hbm}
+    else:
+        roof = dict(hbm)
+        roof["bytes_per_cell"] = bpc
+    roof.update({"kernel": KERNEL[path], "kernel_avg_ms": round(k_ms, 5),
+                 "kernel_launches": k_n})
     return {
         "mlups": cells * steps / el / 1e6,
         "ms_per_step": el * 1e3 / steps,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": _traffic_from_profile(path), "kernel": KERNEL[path],
-                     "kernel_avg_ms": round(k_ms, 5), "kernel_launches": k_n,
-                     "bytes_per_cell": bpc},
+        "roofline": roof,
         "coarse_T_kernel_avg_ms": round(t_ms, 5),
         "coarse_T": [kc.NxT, kc.NyT],
         "info": info,
         "copy_ms": copy_ms,
+        "tip_advance_cells": x1 - x0,
+        "train": train,
+        "fallback": fb,
     }
+
+
+def _tip_x(ctx) -> float:
+    """Tip position (cells, lab frame) from the quartic/parabola fit."""
+    t = ctx.tip()
+    return ctx.constants().i_offset_history + t["x"]
 
 
 def main() -> int:
@@ -118,11 +158,13 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--grid", type=int, default=4096)
-    ap.add_argument("--path", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--path", default="surrogate", choices=["surrogate", "mixed", "fp64"])
+    ap.add_argument("--only", action="store_true", help="time the headline path only")
+    ap.add_argument("--train-epochs", type=int, default=None,
+                    help="surrogate training epochs (default: the library's); profiler runs "
+                         "that only need the step kernel's counters use 1")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64-reference", action="store_true",
-                    help="skip timing the plain FP64 path beside the headline path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
                          "(byte-counter calibration for rocprofv3 --pmc)")
@@ -132,14 +174,19 @@ def main() -> int:
               "(the y-band decomposition is a design note, not code).", file=sys.stderr)
         return 2
 
-    head = run_path(a.path, a.grid, a.steps, a.warmup, a.calibrate_copy)
-    fp64 = None
-    if a.path != "fp64" and not a.no_fp64_reference:
-        fp64 = run_path("fp64", a.grid, max(1, a.steps // 2), a.warmup, False)
-    dtype = {"fp64": "f64", "mixed": "f32+f64"}[a.path]
+    order = [a.path] + ([] if a.only else [q for q in ("surrogate", "mixed", "fp64") if q != a.path])
+    res = {}
+    for path in order:
+        res[path] = run_path(path, a.grid, a.steps, a.warmup,
+                             a.calibrate_copy and path == a.path, a.train_epochs)
+    head = res[a.path]
+    dtype = {"fp64": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64"}[a.path]
     desc = {"fp64": "plain FP64 path",
             "mixed": "FP32 psi/phi and fine T, FP64 c / flux divergence / ln terms / "
-                     "time accumulation, exact CALPHAD (no surrogate)"}[a.path]
+                     "time accumulation, exact CALPHAD (polynomial table)",
+            "surrogate": "FP32 psi/phi and fine T, FP64 c / flux divergence / ln terms / time "
+                         "accumulation, CALPHAD from the BF16-MFMA surrogate fused into the "
+                         "step (exact fallback outside the trust table)"}[a.path]
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 2),
@@ -159,11 +206,29 @@ def main() -> int:
         "roofline": head["roofline"],
         "coarse_T_kernel_avg_ms": head["coarse_T_kernel_avg_ms"],
     }
-    if fp64 is not None:
-        out["plain_fp64"] = {"value": round(fp64["mlups"], 2), "unit": "MLUPS",
-                             "ms_per_step": round(fp64["ms_per_step"], 5),
-                             "roofline": fp64["roofline"]}
-        out["speedup_vs_plain_fp64"] = round(head["mlups"] / fp64["mlups"], 3)
+    out["paths"] = {q: {"value": round(r["mlups"], 2), "unit": "MLUPS",
+                        "ms_per_step": round(r["ms_per_step"], 5),
+                        "kernel_avg_ms": r["roofline"]["kernel_avg_ms"],
+                        "tip_advance_cells": round(r["tip_advance_cells"], 5)}
+                    for q, r in res.items()}
+    if "fp64" in res:
+        ref = res["fp64"]
+        for q, r in res.items():
+            out["paths"][q]["speedup_vs_plain_fp64"] = round(r["mlups"] / ref["mlups"], 3)
+            if q != "fp64" and abs(ref["tip_advance_cells"]) > 0:
+                out["paths"][q]["tip_velocity_err"] = round(
+                    abs(r["tip_advance_cells"] - ref["tip_advance_cells"])
+                    / abs(ref["tip_advance_cells"]), 5)
+        out["speedup_vs_plain_fp64"] = out["paths"][a.path]["speedup_vs_plain_fp64"]
+        if "tip_velocity_err" in out["paths"][a.path]:
+            out["tip_velocity_err"] = out["paths"][a.path]["tip_velocity_err"]
+    if "surrogate" in res:
+        r = res["surrogate"]
+        out["surrogate"] = {"fallback_frac": r["fallback"]["fallback_frac"],
+                            "train_s": round(r["train"]["train_s"], 3),
+                            "train_steps": int(r["train"]["steps"]),
+                            "trusted_bin_frac": r["train"]["trusted_frac"],
+                            "max_residual_trusted": r["train"]["max_res_trusted"]}
     if head["copy_ms"] is not None:
         nb = 2 * 8 * a.grid * a.grid
         out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(head["copy_ms"], 5),

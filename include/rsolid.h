@@ -48,8 +48,11 @@ enum {
 /* ---- precision / thermodynamics paths ---------------------------------- */
 enum {
     RS_PREC_FP64 = 0,     /* plain FP64 path: psi, c, T in FP64              */
-    RS_PREC_MIXED = 1     /* psi/phi and T in FP32; c, flux divergence,
+    RS_PREC_MIXED = 1,    /* psi/phi and T in FP32; c, flux divergence,
                              mass and ln-terms in FP64                        */
+    RS_PREC_SURROGATE = 2 /* RS_PREC_MIXED storage with the learned CALPHAD
+                             surrogate fused into the step kernel; needs
+                             rs_surrogate_train or rs_surrogate_set first    */
 };
 
 /* ---- fields for rs_upload / rs_download -------------------------------- */
@@ -240,6 +243,68 @@ int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
  * byte counters can be calibrated against a known traffic. Destroys the
  * field contents; writes the average kernel time (HIP events). */
 int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms);
+
+/* ---- learned CALPHAD surrogate ------------------------------------------
+ * No counterpart in the reference, which evaluates Gl/Gs/Glc/Gsc
+ * (fun.cu:1671-1720) exactly at every use. A 2 -> 64 -> 64 -> 4 MLP maps
+ * (T, x_Nb = 1 - c) to (f_L, f_S, mu_L, mu_S) without their ideal-mixing
+ * terms, which stay exact in FP64. It is trained on the GPU (BF16 MFMA,
+ * FP32 accumulate, Adam on FP32 master weights) from samples the exact
+ * routine generates, and evaluated inside the RS_PREC_SURROGATE step
+ * kernel. A trust table of 64 x 64 bins over the normalised (T, x_Nb)
+ * domain, built by comparing the network with the exact routine on a
+ * 513 x 513 grid, marks the bins whose nondimensional residual
+ *   max(|d(f_L - f_S)| / (2 hh v0), |d mu_L| / (h0 v0), |d mu_S| / (h0 v0))
+ * stays within residual_tol; cells outside the domain or in an untrusted
+ * bin fall back to the exact routine. */
+typedef struct rs_surrogate_opts {
+    int64_t n_uniform;     /* uniform samples over the domain (2^20)          */
+    int64_t n_near;        /* samples within +-0.05 x_Nb of the liquidus and
+                              solidus compositions (2^18)                     */
+    uint64_t seed;
+    int32_t epochs;        /* passes over the sample set                      */
+    int32_t batch;         /* samples per optimiser step, a multiple of 64    */
+    double T_lo, T_hi;     /* training window, K; T_lo >= T_hi derives it from
+                              the run's temperature field                    */
+    double lr, lr_final;   /* Adam step size, cosine-annealed to lr_final     */
+    double residual_tol;   /* trust bound on the residual above (1e-4)        */
+    int32_t n_curve;       /* points of the liquidus/solidus tables           */
+    int32_t split_act;     /* 1: hidden activations enter the next product as
+                              two BF16 parts (hi + lo), 0: one BF16 part     */
+} rs_surrogate_opts;
+
+#define RS_SURR_NPARAM 4612  /* W1[64][2] b1[64] W2[64][64] b2[64] W3[4][64] b3[4] */
+#define RS_SURR_NMETA 14     /* t_mid, t_half, y_mean[4], y_std[4], T_lo, T_hi, tol,
+                                split_act */
+#define RS_SURR_NREPORT 12
+
+int rs_surrogate_defaults(rs_surrogate_opts* o);
+/* Generates the samples, trains, and builds the trust table. report (up to
+ * RS_SURR_NREPORT): [0] final mean squared error (normalised outputs, whole
+ * sample set), [1] fraction of trusted bins, [2..5] largest |error| (J/mol)
+ * of y0 = f_L - f_S, y1 = f_S - ideal, y2 = mu_L - ideal, y3 = mu_S - mu_L
+ * over the validation grid, [6] largest residual over trusted bins,
+ * [7] training time (s, HIP events), [8] optimiser steps, [9] T_lo,
+ * [10] T_hi, [11] mean squared error before training. */
+int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, int32_t n);
+/* Current master weights (FP32, RS_SURR_NPARAM) and meta (RS_SURR_NMETA). */
+int rs_surrogate_get(rs_ctx* ctx, float* params, int32_t np, double* meta, int32_t nm);
+/* Installs weights and meta, repacks the BF16 fragments and rebuilds the
+ * trust table. */
+int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double* meta, int32_t nm);
+/* Gradient of the mean squared error over n rows (n a multiple of 64,
+ * n <= 65536) of normalised inputs X (n x 2) and targets Yn (n x 4) at the
+ * current weights, by the training kernels; grad in the parameter layout. */
+int rs_surrogate_grad(rs_ctx* ctx, const float* X, const float* Yn, int32_t n, float* grad,
+                      double* loss);
+/* Network evaluation at n (T, c) pairs (Tc: n x 2): out (n x 5) = f_L, f_S,
+ * mu_L, mu_S (J/mol, ideal terms added exactly) and 1.0 where the point is
+ * in the domain and its bin trusted, else 0.0. */
+int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out);
+/* Fallback counts of the fused kernel since the last reset: out[0] cell
+ * evaluations by the exact routine, out[1] all evaluations, out[2] their
+ * ratio. reset != 0 clears the counters afterwards. */
+int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset);
 
 #ifdef __cplusplus
 }
