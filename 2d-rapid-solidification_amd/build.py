@@ -37,9 +37,10 @@ PER_FILE = {
     # contraction, so duplicated face-flux evaluations round identically
     "rs_mixed.hip": ["-ffp-contract=off"],
     "rs_surr.hip": ["-ffp-contract=off"],
+    "rs_batch.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
-           "rs_capi.hip", "rs_surr.hip"]
+           "rs_capi.hip", "rs_surr.hip", "rs_batch.hip"]
 
 LIB = os.path.join(PKG, "rsolid", "librsolid.so")
 ORACLE_SRC = os.path.join(ROOT, "oracle", "rs_oracle.c")

@@ -1,0 +1,548 @@
+// rs_batch.hip -- batched ensemble: many independent runs (uneven grid
+// sizes and undercoolings) advanced by one persistent step launch per step,
+// the step sequence captured in a HIP graph, tile shapes from an autotuner.
+// Each run is an ordinary context (its own buffers, init, download, tip,
+// pull-back); the batch adds the device-side run table and the per-run
+// state that the captured sequence advances without the host.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rs_host.hpp"
+
+using namespace rs;
+
+struct rs_batch {
+    std::vector<rs_ctx*> runs;
+    hipStream_t stream = nullptr;
+    BatchRun* d_runs = nullptr;
+    BatchDyn* d_dyn = nullptr;
+    int* d_item0 = nullptr;
+    int* d_cell0 = nullptr;
+    std::vector<BatchRun> h_runs;
+    std::vector<BatchDyn> h_dyn;
+    std::vector<int> band_h;     // per run (autotuned)
+    BatchLaunch L{};
+    int cus = 256;
+    int grid_mult = 1;           // persistent blocks = resident * CUs * grid_mult / 1
+    int graph_steps = 8;         // steps per captured graph
+    hipGraphExec_t g1 = nullptr, gk = nullptr;
+    bool dirty = true;           // launch shape changed: re-capture
+    std::string err;
+    double last_ms = 0.0;
+    int64_t last_steps = 0;
+    std::vector<double> tune_log;
+};
+
+namespace {
+
+int bfail(rs_batch* b, int code, const char* what, hipError_t e = hipSuccess) {
+    char buf[512];
+    if (e != hipSuccess)
+        snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+    else
+        snprintf(buf, sizeof buf, "%s", what);
+    b->err = buf;
+    return code;
+}
+
+#define BCHK(b, call)                                                   \
+    do {                                                                \
+        hipError_t e_ = (call);                                         \
+        if (e_ != hipSuccess) return bfail((b), RS_ERR_HIP, #call, e_); \
+    } while (0)
+
+void drop_graphs(rs_batch* b) {
+    if (b->g1) (void)hipGraphExecDestroy(b->g1);
+    if (b->gk) (void)hipGraphExecDestroy(b->gk);
+    b->g1 = b->gk = nullptr;
+}
+
+// static per-run descriptors and dynamic state from the contexts' host state
+int build_tables(rs_batch* b) {
+    const int n = (int)b->runs.size();
+    b->h_runs.resize(n);
+    b->h_dyn.resize(n);
+    std::vector<int> item0(n + 1, 0), cell0(n + 1, 0);
+    bool ident = true;
+    for (int r = 0; r < n; ++r) {
+        rs_ctx* c = b->runs[r];
+        c->band_h = b->band_h[r];
+        const rs_params& p = c->p;
+        const rs_constants& k = c->k;
+        BatchRun& R = b->h_runs[r];
+        R.a = mixed_step_args(c);
+        if (c->surrogate) surr_step_args(c, R.a);
+        R.a.band_h = b->band_h[r];
+        R.a.nbands = (p.Ny - 2 + R.a.band_h - 1) / R.a.band_h;
+        for (int q = 0; q < 2; ++q) {
+            R.psi[q] = c->psi32[q];
+            R.c[q] = c->c[q];
+            R.T32[q] = c->T32[q];
+            R.T[q] = c->T[q];
+        }
+        R.t = coarse_T_args(c, k.step);
+        R.ntx = (p.Nx - 2 + kTileOut - 1) / kTileOut;
+        R.nbg = (R.a.nbands + b->L.wpb - 1) / b->L.wpb;
+        R.tfc = p.ifTFC;
+        R.ncT = p.ifTFC ? k.NxT * k.NyT : 0;
+        R.Vp = p.Vp;
+        R.Vp_rate = p.Vp_ramp_rate;
+        R.ramp = p.if_Vp_ramp;
+        R.tau0 = tau0_value(p);
+        R.dt = k.dt;
+        R.S = (double)p.S;
+        R.W0 = p.W0;
+        R.SW0 = p.S * p.W0;
+        R.ioff = -p.i_tip_target + k.i_offset_history - k.i_offset_history_initial;
+        R.Tmelt = p.Tmelt;
+        R.deltaT0 = p.deltaT0;
+        R.delta_initial = k.delta_initial;
+        R.GG = p.GG;
+        R.l_off0 = k.l_offset_Vp_initial;
+        R.gstep = p.GG * p.dx * p.S * p.W0;
+        BatchDyn& d = b->h_dyn[r];
+        d.step = k.step;
+        d.l_offset_Vp = k.l_offset_Vp;
+        d.cur = c->cur;
+        d.tcur = c->tcur;
+        d.ile_interp = k.i_left_end_interp;
+        d.pad = 0;
+        item0[r + 1] = item0[r] + R.ntx * R.nbg;
+        cell0[r + 1] = cell0[r] + R.ncT;
+        const double* o = c->orient;
+        ident = ident && o[0] == 1.0 && o[1] == 0.0 && o[3] == 0.0 && o[4] == 1.0 && o[6] == 0.0 &&
+                o[7] == 0.0;
+    }
+    BCHK(b, hipMemcpyAsync(b->d_runs, b->h_runs.data(), sizeof(BatchRun) * n, hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_dyn, b->h_dyn.data(), sizeof(BatchDyn) * n, hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_item0, item0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_cell0, cell0.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice,
+                           b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    BatchLaunch& L = b->L;
+    L.runs = b->d_runs;
+    L.dyn = b->d_dyn;
+    L.item0 = b->d_item0;
+    L.cell0 = b->d_cell0;
+    L.nruns = n;
+    L.items = item0[n];
+    L.cells = cell0[n];
+    L.ident = ident;
+    const rs_ctx* c0 = b->runs[0];
+    L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
+    if (L.surr) L.surr = b->h_runs[0].a.s_split ? 2 : 1;
+    const int res = batch_step_resident(L);
+    L.step_blocks = std::max(1, std::min(L.items, res * b->cus * b->grid_mult));
+    L.T_blocks = std::max(1, std::min((L.cells + 255) / 256, 8 * b->cus));
+    return RS_OK;
+}
+
+// contexts' host state from the device-side per-run state
+int pull_state(rs_batch* b) {
+    const int n = (int)b->runs.size();
+    BCHK(b, hipMemcpyAsync(b->h_dyn.data(), b->d_dyn, sizeof(BatchDyn) * n, hipMemcpyDeviceToHost,
+                           b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (int r = 0; r < n; ++r) {
+        rs_ctx* c = b->runs[r];
+        const BatchDyn& d = b->h_dyn[r];
+        c->k.step = d.step;
+        c->k.l_offset_Vp = d.l_offset_Vp;
+        c->cur = d.cur;
+        c->tcur = d.tcur;
+        c->k.i_left_end_interp = d.ile_interp;
+    }
+    return RS_OK;
+}
+
+int capture(rs_batch* b) {
+    drop_graphs(b);
+    for (int g = 0; g < 2; ++g) {
+        const int nst = g == 0 ? 1 : b->graph_steps;
+        hipGraph_t graph;
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int q = 0; q < nst && e == hipSuccess; ++q) e = launch_batch_step(b->L, b->stream);
+        hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+        if (e != hipSuccess) return bfail(b, RS_ERR_HIP, "batch step capture", e);
+        if (e2 != hipSuccess) return bfail(b, RS_ERR_HIP, "hipStreamEndCapture", e2);
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return bfail(b, RS_ERR_HIP, "hipGraphInstantiate", e);
+        (g == 0 ? b->g1 : b->gk) = ex;
+    }
+    b->dirty = false;
+    return RS_OK;
+}
+
+// steps until the next pull-back of any run (at least 1)
+int64_t steps_to_pull_back(rs_batch* b) {
+    int64_t m = INT64_MAX;
+    for (rs_ctx* c : b->runs) {
+        const int64_t spb = c->k.step_pull_back;
+        const int64_t left = spb - c->k.step % spb;
+        m = std::min(m, left);
+    }
+    return std::max<int64_t>(m, 1);
+}
+
+int run_segment(rs_batch* b, int64_t n) {
+    int rc = build_tables(b);
+    if (rc) return rc;
+    if (b->dirty || !b->g1) {
+        rc = capture(b);
+        if (rc) return rc;
+    }
+    int64_t left = n;
+    while (left >= b->graph_steps) {
+        BCHK(b, hipGraphLaunch(b->gk, b->stream));
+        left -= b->graph_steps;
+    }
+    while (left-- > 0) BCHK(b, hipGraphLaunch(b->g1, b->stream));
+    return pull_state(b);
+}
+
+int do_steps(rs_batch* b, int64_t nsteps) {
+    for (rs_ctx* c : b->runs) BCHK(b, hipStreamSynchronize(c->stream));
+    int64_t done = 0;
+    while (done < nsteps) {
+        // pull-backs due at the runs' current step (fun.cu:448, includes step 0)
+        for (rs_ctx* c : b->runs) {
+            if (c->k.step % c->k.step_pull_back == 0) {
+                int32_t it = 0;
+                int rc = rs_tip_column(c, &it);
+                if (!rc) rc = rs_pull_back(c, it - c->p.i_tip_target);
+                if (rc) return bfail(b, rc, c->err.c_str());
+            }
+        }
+        const int64_t seg = std::min(nsteps - done, steps_to_pull_back(b));
+        int rc = run_segment(b, seg);
+        if (rc) return rc;
+        done += seg;
+    }
+    return RS_OK;
+}
+
+// device copies of every run's evolving buffers (tuning runs on them)
+struct Snapshot {
+    std::vector<std::pair<void*, void*>> bufs;  // (saved, live)
+    std::vector<size_t> bytes;
+    std::vector<BatchDyn> dyn;
+    ~Snapshot() {
+        for (auto& q : bufs) (void)hipFree(q.first);
+    }
+};
+
+int snapshot(rs_batch* b, Snapshot& s, bool restore) {
+    if (!restore) {
+        for (rs_ctx* c : b->runs) {
+            const size_t f32 = sizeof(float) * (size_t)c->n, f64 = sizeof(double) * (size_t)c->n;
+            const size_t tb = sizeof(double) * (size_t)(c->nT + c->k.NxT_all);
+            const size_t tb32 = sizeof(float) * (size_t)(c->nT + c->k.NxT_all);
+            const size_t lhb = sizeof(double) * (size_t)c->p.Nx * (size_t)(c->k.NyT > 2 ? c->k.NyT - 2 : 1);
+            void* live[] = {c->psi32[0], c->psi32[1], c->c[0], c->c[1], c->T[0], c->T[1],
+                            c->T32[0], c->T32[1], c->lh};
+            const size_t sz[] = {f32, f32, f64, f64, tb, tb, tb32, tb32, lhb};
+            for (int q = 0; q < 9; ++q) {
+                void* sv = nullptr;
+                BCHK(b, hipMalloc(&sv, sz[q]));
+                s.bufs.emplace_back(sv, live[q]);
+                s.bytes.push_back(sz[q]);
+                BCHK(b, hipMemcpyAsync(sv, live[q], sz[q], hipMemcpyDeviceToDevice, b->stream));
+            }
+        }
+        s.dyn.resize(b->runs.size());
+        for (size_t r = 0; r < b->runs.size(); ++r) {
+            rs_ctx* c = b->runs[r];
+            s.dyn[r] = BatchDyn{c->k.step, c->k.l_offset_Vp, c->cur, c->tcur, c->k.i_left_end_interp, 0};
+        }
+    } else {
+        for (size_t q = 0; q < s.bufs.size(); ++q)
+            BCHK(b, hipMemcpyAsync(s.bufs[q].second, s.bufs[q].first, s.bytes[q], hipMemcpyDeviceToDevice,
+                                   b->stream));
+        for (size_t r = 0; r < b->runs.size(); ++r) {
+            rs_ctx* c = b->runs[r];
+            c->k.step = s.dyn[r].step;
+            c->k.l_offset_Vp = s.dyn[r].l_offset_Vp;
+            c->cur = s.dyn[r].cur;
+            c->tcur = s.dyn[r].tcur;
+            c->k.i_left_end_interp = s.dyn[r].ile_interp;
+        }
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    return RS_OK;
+}
+
+// timed graph replays of the current launch shape (ms per step)
+int time_shape(rs_batch* b, int steps, double* ms) {
+    b->dirty = true;
+    int rc = build_tables(b);
+    if (!rc) rc = capture(b);
+    if (rc) return rc;
+    BCHK(b, hipGraphLaunch(b->g1, b->stream));  // warm-up
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    BCHK(b, hipEventRecord(e0, b->stream));
+    for (int q = 0; q < steps; ++q) BCHK(b, hipGraphLaunch(b->g1, b->stream));
+    BCHK(b, hipEventRecord(e1, b->stream));
+    BCHK(b, hipEventSynchronize(e1));
+    float t = 0.f;
+    BCHK(b, hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / steps;
+    return RS_OK;
+}
+
+// tuned band heights per (precision, grid) shared by batches of the process
+std::map<std::pair<int, std::pair<int, int>>, int>& tuned_bands() {
+    static std::map<std::pair<int, std::pair<int, int>>, int> m;
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out) {
+    if (!params || !out || n < 1) return RS_ERR_ARG;
+    *out = nullptr;
+    for (int r = 0; r < n; ++r) {
+        if (params[r].precision != RS_PREC_MIXED && params[r].precision != RS_PREC_SURROGATE)
+            return RS_ERR_ARG;  // the batched kernels are the FP32-field kernels
+        if (params[r].precision != params[0].precision) return RS_ERR_ARG;
+    }
+    rs_batch* b = new (std::nothrow) rs_batch;
+    if (!b) return RS_ERR_NOMEM;
+    for (int r = 0; r < n; ++r) {
+        rs_ctx* c = nullptr;
+        const int rc = rs_create(&params[r], &c);
+        if (rc) {
+            for (rs_ctx* q : b->runs) rs_destroy(q);
+            delete b;
+            return rc;
+        }
+        if (c->p.ifTFC && c->k.ratio_dt_dtT != 1) {
+            rs_destroy(c);
+            for (rs_ctx* q : b->runs) rs_destroy(q);
+            delete b;
+            return RS_ERR_ARG;  // one coarse-T sub-step per step in the captured sequence
+        }
+        b->runs.push_back(c);
+        const auto key = std::make_pair(params[r].precision, std::make_pair(params[r].Nx, params[r].Ny));
+        const auto it = tuned_bands().find(key);
+        b->band_h.push_back(it != tuned_bands().end() ? it->second : c->band_h);
+    }
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        b->cus = prop.multiProcessorCount;
+    b->L.wpb = 4;
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&b->d_runs, sizeof(BatchRun) * n) == hipSuccess &&
+              hipMalloc(&b->d_dyn, sizeof(BatchDyn) * n) == hipSuccess &&
+              hipMalloc(&b->d_item0, sizeof(int) * (n + 1)) == hipSuccess &&
+              hipMalloc(&b->d_cell0, sizeof(int) * (n + 1)) == hipSuccess;
+    if (!ok) {
+        rs_batch_destroy(b);
+        return RS_ERR_NOMEM;
+    }
+    *out = b;
+    return RS_OK;
+}
+
+int rs_batch_destroy(rs_batch* b) {
+    if (!b) return RS_ERR_ARG;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graphs(b);
+    void* ps[] = {b->d_runs, b->d_dyn, b->d_item0, b->d_cell0};
+    for (void* q : ps)
+        if (q) (void)hipFree(q);
+    for (rs_ctx* c : b->runs) rs_destroy(c);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return RS_OK;
+}
+
+const char* rs_batch_last_error(const rs_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int rs_batch_run(rs_batch* b, int32_t i, rs_ctx** ctx) {
+    if (!b || !ctx || i < 0 || i >= (int)b->runs.size()) return RS_ERR_ARG;
+    *ctx = b->runs[i];
+    return RS_OK;
+}
+
+int rs_batch_step(rs_batch* b, int64_t nsteps) {
+    if (!b || nsteps < 0) return RS_ERR_ARG;
+    for (rs_ctx* c : b->runs)
+        if (c->surrogate && !surr_ready(c)) return bfail(b, RS_ERR_STATE, "surrogate path needs trained weights");
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    BCHK(b, hipEventRecord(e0, b->stream));
+    int rc = do_steps(b, nsteps);
+    if (!rc) {
+        BCHK(b, hipEventRecord(e1, b->stream));
+        BCHK(b, hipEventSynchronize(e1));
+        float t = 0.f;
+        BCHK(b, hipEventElapsedTime(&t, e0, e1));
+        b->last_ms = t;
+        b->last_steps = nsteps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int rs_batch_surrogate_train(rs_batch* b, const rs_surrogate_opts* o, double* report, int32_t n) {
+    if (!b || !o) return RS_ERR_ARG;
+    if (!b->runs[0]->surrogate) return bfail(b, RS_ERR_STATE, "batch is not on the surrogate path");
+    rs_surrogate_opts oo = *o;
+    if (!(oo.T_lo < oo.T_hi)) {
+        // one network for every run: the union of the runs' windows, kept on
+        // one side of the 1941 K G_Ti breakpoint like a single run's -- the
+        // side holding more of the runs' cell-weighted temperature range
+        const double brk = 1941.0;
+        double lo = 1e30, hi = -1e30, below = 0.0, above = 0.0;
+        for (rs_ctx* c : b->runs) {
+            const double tl = (c->p.ifTFC ? c->k.Tcold : c->p.Tmelt - c->p.deltaT0) - 20.0;
+            const double th = (c->p.ifTFC ? c->k.Thot : c->p.Tmelt) + 25.0;
+            lo = std::min(lo, tl);
+            hi = std::max(hi, th);
+            const double w = (double)c->n / std::max(th - tl, 1e-9);
+            below += w * std::max(0.0, std::min(th, brk) - tl);
+            above += w * std::max(0.0, th - std::max(tl, brk));
+        }
+        oo.T_lo = lo;
+        oo.T_hi = hi;
+        if (lo < brk && hi > brk) {
+            if (below >= above)
+                oo.T_hi = brk;
+            else
+                oo.T_lo = brk;
+        }
+    }
+    int rc = rs_surrogate_train(b->runs[0], &oo, report, n);
+    if (rc) return bfail(b, rc, b->runs[0]->err.c_str());
+    std::vector<float> P(RS_SURR_NPARAM);
+    double meta[RS_SURR_NMETA];
+    rc = rs_surrogate_get(b->runs[0], P.data(), RS_SURR_NPARAM, meta, RS_SURR_NMETA);
+    for (size_t r = 1; r < b->runs.size() && !rc; ++r)
+        rc = rs_surrogate_set(b->runs[r], P.data(), RS_SURR_NPARAM, meta, RS_SURR_NMETA);
+    if (rc) return bfail(b, rc, "installing the shared surrogate");
+    b->dirty = true;
+    return RS_OK;
+}
+
+// Autotuner: band height (rows per wavefront tile, a multiple of the 10-row
+// latent-heat blocks) per distinct grid of the batch by coordinate descent,
+// then the persistent grid size, each candidate timed over graph replays on
+// a snapshot of the state, which is restored afterwards. Results are cached
+// per (precision, grid) for later batches of the process.
+int rs_batch_tune(rs_batch* b, int32_t steps, double* out, int32_t nout) {
+    if (!b || steps < 1) return RS_ERR_ARG;
+    for (rs_ctx* c : b->runs)
+        if (c->surrogate && !surr_ready(c)) return bfail(b, RS_ERR_STATE, "surrogate path needs trained weights");
+    for (rs_ctx* c : b->runs) BCHK(b, hipStreamSynchronize(c->stream));
+    Snapshot snap;
+    int rc = snapshot(b, snap, false);
+    if (rc) return rc;
+    b->tune_log.clear();
+    std::vector<std::pair<int, int>> grids;
+    for (rs_ctx* c : b->runs) {
+        const auto g = std::make_pair(c->p.Nx, c->p.Ny);
+        if (std::find(grids.begin(), grids.end(), g) == grids.end()) grids.push_back(g);
+    }
+    const int cand[] = {10, 20, 30, 40, 60, 80};
+    double best = 0.0;
+    rc = time_shape(b, steps, &best);
+    if (!rc) rc = snapshot(b, snap, true);
+    for (const auto& g : grids) {
+        if (rc) break;
+        std::vector<int> saved = b->band_h;
+        int keep = -1;
+        for (int bh : cand) {
+            bool any = false;
+            for (size_t r = 0; r < b->runs.size(); ++r)
+                if (b->runs[r]->p.Nx == g.first && b->runs[r]->p.Ny == g.second && bh <= b->runs[r]->p.Ny - 2) {
+                    b->band_h[r] = bh;
+                    any = true;
+                }
+            if (!any) continue;
+            double ms = 0.0;
+            rc = time_shape(b, steps, &ms);
+            if (!rc) rc = snapshot(b, snap, true);
+            if (rc) break;
+            b->tune_log.push_back((double)g.first);
+            b->tune_log.push_back((double)bh);
+            b->tune_log.push_back(ms);
+            if (ms < best) {
+                best = ms;
+                keep = bh;
+            }
+        }
+        if (rc) break;
+        for (size_t r = 0; r < b->runs.size(); ++r)
+            if (b->runs[r]->p.Nx == g.first && b->runs[r]->p.Ny == g.second)
+                b->band_h[r] = keep > 0 ? keep : saved[r];
+    }
+    int best_mult = b->grid_mult;
+    for (int m : {1, 2, 4}) {
+        if (rc) break;
+        b->grid_mult = m;
+        double ms = 0.0;
+        rc = time_shape(b, steps, &ms);
+        if (!rc) rc = snapshot(b, snap, true);
+        if (!rc && ms < best) {
+            best = ms;
+            best_mult = m;
+        }
+    }
+    b->grid_mult = best_mult;
+    b->dirty = true;
+    if (rc) return rc;
+    for (size_t r = 0; r < b->runs.size(); ++r) {
+        const rs_params& p = b->runs[r]->p;
+        tuned_bands()[std::make_pair(p.precision, std::make_pair(p.Nx, p.Ny))] = b->band_h[r];
+    }
+    if (out) {
+        // [0] best ms per step, [1] grid multiplier, then (Nx, band_h) per grid
+        std::vector<double> v = {best, (double)b->grid_mult};
+        for (const auto& g : grids)
+            for (size_t r = 0; r < b->runs.size(); ++r)
+                if (b->runs[r]->p.Nx == g.first && b->runs[r]->p.Ny == g.second) {
+                    v.push_back((double)g.first);
+                    v.push_back((double)b->band_h[r]);
+                    break;
+                }
+        for (int q = 0; q < nout && q < (int)v.size(); ++q) out[q] = v[q];
+    }
+    return RS_OK;
+}
+
+int rs_batch_info(rs_batch* b, double* out, int32_t n) {
+    if (!b || !out) return RS_ERR_ARG;
+    int64_t cells = 0;
+    for (rs_ctx* c : b->runs) cells += (int64_t)c->p.Nx * c->p.Ny;
+    int rc = build_tables(b);
+    if (rc) return rc;
+    const double v[9] = {(double)b->runs.size(), (double)cells, (double)b->L.items,
+                         (double)b->L.step_blocks, (double)b->L.wpb, (double)b->graph_steps,
+                         b->last_ms, (double)b->last_steps, (double)b->grid_mult};
+    for (int q = 0; q < n && q < 9; ++q) out[q] = v[q];
+    return RS_OK;
+}
+
+}  // extern "C"

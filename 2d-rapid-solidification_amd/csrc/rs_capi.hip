@@ -261,6 +261,36 @@ int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
     return check_window(ctx, ctx->k.i_left_end);
 }
 
+// coarse-T kernel arguments for the step `step` (the T/Tn/T32n pointers
+// are set by the caller per sub-step)
+TempArgs temp_args(rs_ctx* ctx, int64_t step) {
+    const rs_params& p = ctx->p;
+    const rs_constants& k = ctx->k;
+    const double tau0 = tau0_of(p);
+    TempArgs t;
+    t.lh = ctx->lh;
+    t.Nx = p.Nx;
+    t.NxT = k.NxT;
+    t.NyT = k.NyT;
+    t.NxT_all = k.NxT_all;
+    t.ratio = p.ratio_dxT_dx;
+    t.ile = k.i_left_end;
+    t.ratio_dt_dtT = k.ratio_dt_dtT;
+    t.vadv = vp_ramp(p, step * tau0 * k.dt) * tau0 / (p.S * p.W0);
+    t.dxT = p.dx * p.ratio_dxT_dx;
+    t.dtT = k.dtT;
+    t.DT = p.DT;
+    t.Gmu = p.Gamma * p.muk0;
+    t.Lcp = ctx->d.Lcp;
+    t.Tcold = k.Tcold;
+    t.Thot = k.Thot;
+    t.Tref = kMixedTref;
+    t.T = nullptr;
+    t.Tn = nullptr;
+    t.T32n = nullptr;
+    return t;
+}
+
 uint64_t splitmix64(uint64_t* s) {
     uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -269,6 +299,12 @@ uint64_t splitmix64(uint64_t* s) {
 }
 
 }  // namespace
+
+namespace rs {
+StepArgsMixed mixed_step_args(rs_ctx* ctx) { return step_args_mixed(ctx); }
+TempArgs coarse_T_args(rs_ctx* ctx, int64_t step) { return temp_args(ctx, step); }
+double tau0_value(const rs_params& p) { return tau0_of(p); }
+}  // namespace rs
 
 extern "C" {
 
@@ -553,24 +589,7 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             if (rc) return rc;
         }
         if (p.ifTFC) {
-            TempArgs t;
-            t.lh = ctx->lh;
-            t.Nx = p.Nx;
-            t.NxT = k.NxT;
-            t.NyT = k.NyT;
-            t.NxT_all = k.NxT_all;
-            t.ratio = p.ratio_dxT_dx;
-            t.ile = k.i_left_end;
-            t.ratio_dt_dtT = k.ratio_dt_dtT;
-            t.vadv = vp_ramp(p, step * tau0 * k.dt) * tau0 / (p.S * p.W0);
-            t.dxT = p.dx * p.ratio_dxT_dx;
-            t.dtT = k.dtT;
-            t.DT = p.DT;
-            t.Gmu = p.Gamma * p.muk0;
-            t.Lcp = ctx->d.Lcp;
-            t.Tcold = k.Tcold;
-            t.Thot = k.Thot;
-            t.Tref = kMixedTref;
+            TempArgs t = temp_args(ctx, step);
             if (ctx->timing) {
                 int rc = ev_record(ctx, 1);
                 if (rc) return rc;

@@ -23,6 +23,11 @@ struct TipFit {
 };
 int precise_tip(const rs_params& p, const double* psi, TipFit* out);
 
+// step arguments of a context at its current state (rs_capi.hip)
+StepArgsMixed mixed_step_args(rs_ctx* ctx);
+TempArgs coarse_T_args(rs_ctx* ctx, int64_t step);
+double tau0_value(const rs_params& p);
+
 // learned CALPHAD surrogate (rs_surr.hip)
 struct SurrState;
 void surr_free(rs_ctx* ctx);

@@ -53,6 +53,53 @@ struct TempArgs {
     double Tref;
 };
 
+// integrate_phi (fun.cu:1494-1535) from the column sums + compute_temperature
+// (fun.cu:1537-1578) for coarse cell (iT, jT) < (NxT, NyT), race-free
+// boundaries: the owner of row NyT-2 writes row 0, the owner of row 1 writes
+// row NyT-1, Dirichlet columns by their own threads
+__device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int jT) {
+    const int64_t P = a.NxT_all;
+    if (iT == 0 || iT == a.NxT - 1) {
+        const double v = (iT == 0) ? a.Tcold : a.Thot;
+        a.Tn[jT * P + iT] = v;
+        if (a.T32n) a.T32n[jT * P + iT] = (float)(v - a.Tref);
+        return;
+    }
+    if (jT == 0 || jT == a.NyT - 1) return;
+    const int r = a.ratio;
+    const int iT1 = a.ile / r;
+    const int iTn = (a.ile + a.Nx - 3) / r;
+    double dphidt = 0.0;
+    if (iT >= iT1 && iT <= iTn) {
+        int i0 = r * iT + 1 - a.ile;
+        int in = i0 + r - 1;
+        if (i0 < 1) i0 = 1;
+        if (in > a.Nx - 2) in = a.Nx - 2;
+        const double* row = a.lh + (int64_t)(jT - 1) * a.Nx;
+        double integral = 0.0;
+        for (int i = i0; i <= in; ++i) integral += row[i];
+        dphidt = integral * 0.5;
+    }
+    const double* T = a.T;
+    const double T00 = T[jT * P + iT];
+    const double T10 = T[jT * P + iT + 1];
+    const double Tm0 = T[jT * P + iT - 1];
+    const double T01 = T[(jT + 1) * P + iT];
+    const double T0m = T[(jT - 1) * P + iT];
+    const double lap = (T10 + Tm0 + T01 + T0m - 4.0 * T00) / (a.dxT * a.dxT);
+    const double dTdt = a.vadv * (T10 - Tm0) / (2.0 * a.dxT) + lap * a.DT / a.Gmu;
+    const double tn = T00 + a.dtT * dTdt + a.Lcp * dphidt / (a.ratio_dt_dtT * (r * r));
+    a.Tn[jT * P + iT] = tn;
+    if (jT == a.NyT - 2) a.Tn[iT] = tn;                      // row 0 <- row NyT-2
+    if (jT == 1) a.Tn[(int64_t)(a.NyT - 1) * P + iT] = tn;  // row NyT-1 <- row 1
+    if (a.T32n) {
+        const float t32 = (float)(tn - a.Tref);
+        a.T32n[jT * P + iT] = t32;
+        if (jT == a.NyT - 2) a.T32n[iT] = t32;
+        if (jT == 1) a.T32n[(int64_t)(a.NyT - 1) * P + iT] = t32;
+    }
+}
+
 hipError_t launch_step_fp64(const StepArgsF64& a, int waves_per_block, hipStream_t s);
 hipError_t launch_temperature(const TempArgs& a, hipStream_t s);
 hipError_t launch_init_T(double* T0, double* T1, int NxT_all, int NyT_all, double Tcold, int S,

@@ -207,7 +207,7 @@ struct SurrLds {
 template <bool SPLIT>
 __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* lntab,
                                          const SurrLds& sl, float Tp, double c, float h, int i,
-                                         int j, float& dG, double& mu, unsigned& nfb,
+                                         int j, bool real, float& dG, double& mu, unsigned& nfb,
                                          unsigned& nall) {
     const int lane = threadIdx.x;  // blockDim.x == 64
     const float cf = (float)c;
@@ -220,7 +220,6 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
     const MlpInfer& F =
         *reinterpret_cast<const MlpInfer*>(reinterpret_cast<const char*>(sl.F) + zoff);
     const mlp::f4 y = mlp::forward<SPLIT>(F, mlp::input_frag(ain, bin), lane);
-    const bool real = (int)(blockIdx.x * kTileOut + lane) < a.Nx;
     bool ok = fabsf(ain) <= 1.0f && fabsf(bin) <= 1.0f && i > 1 && i < a.Nx - 2 && j > 1 &&
               j < a.Ny - 2;
     if (ok) {
@@ -252,7 +251,7 @@ struct CellM {
 template <int SURR>
 __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const double* lntab,
                                              const SurrLds& sl, const RowT& rt, float psi,
-                                             double c, int i, int j, unsigned& nfb,
+                                             double c, int i, int j, bool real, unsigned& nfb,
                                              unsigned& nall) {
     CellM q;
     q.psi = psi;
@@ -263,7 +262,7 @@ __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const doubl
     const float Tp = temp_m(a, rt, i);
     const float h = 0.5f * (1.0f + gg<float>(psi));
     if constexpr (SURR != 0)
-        thermo_s<SURR == 2>(a, lntab, sl, Tp, c, h, i, j, q.dG, q.mu, nfb, nall);
+        thermo_s<SURR == 2>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb, nall);
     else
         thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
     return q;
@@ -338,41 +337,59 @@ struct Slot {
 template <int SURR>
 __device__ __forceinline__ void fill_slot(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int j, float psi, double c, int ic,
-                                          Slot& s, unsigned& nfb, unsigned& nall) {
-    s.q = make_cell_m<SURR>(a, lntab, sl, row_t(a, j), psi, c, ic, j, nfb, nall);
+                                          bool real, Slot& s, unsigned& nfb, unsigned& nall) {
+    s.q = make_cell_m<SURR>(a, lntab, sl, row_t(a, j), psi, c, ic, j, real, nfb, nall);
     s.phL = shup(s.q.phi);
     s.phR = shdn(s.q.phi);
     s.SR = shdn(s.q.S);
 }
 
-// SURR: 0 exact thermodynamics, 1 surrogate, 2 surrogate with split
-// activations. At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
-// the exact thermodynamics: the kernel is latency-bound below that. The
-// surrogate variants hold the MFMA operands and run at 2.
-template <bool IDENT, int SURR>
-__global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
-    __shared__ double lntab[2 * RS_LN_TAB_N];
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    const int nthr = blockDim.x * blockDim.y;
-    for (int q = tid; q < 2 * RS_LN_TAB_N; q += nthr) lntab[q] = kLnTab[q];
-    SurrLds sl{nullptr, nullptr};
-    if constexpr (SURR != 0) {
-        __shared__ MlpInfer smlp;
-        __shared__ uint64_t strust[kTrustN];
-        const uint4* src = reinterpret_cast<const uint4*>(a.mlp);
-        uint4* dst = reinterpret_cast<uint4*>(&smlp);
-        for (int q = tid; q < (int)(sizeof(MlpInfer) / 16); q += nthr) dst[q] = src[q];
-        for (int q = tid; q < kTrustN; q += nthr) strust[q] = a.trust[q];
-        sl.F = &smlp;
-        sl.trust = strust;
-    }
-    __syncthreads();
-    unsigned nfb = 0, nall = 0;
+// LDS tables of the step: the FP64 log table and, with the surrogate, the
+// network fragments and trust bitmap
+template <int SURR>
+struct StepLds;
+template <>
+struct StepLds<0> {
+    double lntab[2 * RS_LN_TAB_N];
+};
+template <>
+struct StepLds<1> {
+    double lntab[2 * RS_LN_TAB_N];
+    MlpInfer mlp;
+    uint64_t trust[kTrustN];
+};
+template <>
+struct StepLds<2> : StepLds<1> {};
 
+template <int SURR>
+__device__ __forceinline__ void load_step_lds(StepLds<SURR>& L, const MlpInfer* mlp,
+                                              const uint64_t* trust, int tid, int nthr) {
+    for (int q = tid; q < 2 * RS_LN_TAB_N; q += nthr) L.lntab[q] = kLnTab[q];
+    if constexpr (SURR != 0) {
+        const uint4* src = reinterpret_cast<const uint4*>(mlp);
+        uint4* dst = reinterpret_cast<uint4*>(&L.mlp);
+        for (int q = tid; q < (int)(sizeof(MlpInfer) / 16); q += nthr) dst[q] = src[q];
+        for (int q = tid; q < kTrustN; q += nthr) L.trust[q] = trust[q];
+    }
+}
+
+template <int SURR>
+__device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
+    if constexpr (SURR != 0)
+        return SurrLds{&L.mlp, L.trust};
+    else
+        return SurrLds{nullptr, nullptr};
+}
+
+// One wavefront's work: column tile tile_x (62 output columns), band `band`
+// (rows 1 + band*band_h ...), marching down the band with a 3-row window.
+template <bool IDENT, int SURR>
+__device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
+                                          const SurrLds& sl, int tile_x, int band, unsigned& nfb,
+                                          unsigned& nall) {
     const int lane = threadIdx.x;
-    const int band = blockIdx.y * blockDim.y + threadIdx.y;
     if (band >= a.nbands) return;
-    const int i = blockIdx.x * kTileOut + lane;
+    const int i = tile_x * kTileOut + lane;
     const int ic = i < a.Nx ? i : a.Nx - 1;
     const bool own = lane >= 1 && lane <= kTileOut && i <= a.Nx - 2;
     const int jb = 1 + band * a.band_h;
@@ -380,9 +397,11 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
     const int Nx = a.Nx;
 
     Slot s0, s1, s2;
+    const bool real = i < a.Nx;  // lanes past the last column duplicate it
     fill_slot<SURR>(a, lntab, sl, jb - 1, a.psi[(jb - 1) * Nx + ic], a.c[(jb - 1) * Nx + ic], ic,
-                    s0, nfb, nall);
-    fill_slot<SURR>(a, lntab, sl, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, s1, nfb, nall);
+                    real, s0, nfb, nall);
+    fill_slot<SURR>(a, lntab, sl, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, real, s1, nfb,
+                    nall);
     // vertex (i+1/2, jb-1/2) and the y-face flux (i, jb-1/2)
     float Vpm = quad4(s0.q.S, s0.SR, s1.q.S, s1.SR);
     const float Vmm = shup(Vpm);
@@ -400,7 +419,7 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
             nx_psi = a.psi[(j + 2) * Nx + ic];
             nx_c = a.c[(j + 2) * Nx + ic];
         }
-        fill_slot<SURR>(a, lntab, sl, j + 1, cur_psi, cur_c, ic, P, nfb, nall);
+        fill_slot<SURR>(a, lntab, sl, j + 1, cur_psi, cur_c, ic, real, P, nfb, nall);
         const double muR_z = shdn(Z.q.mu);
         // vertex (i+1/2, j+1/2), its left neighbour, and the faces
         const float Vpp = quad4(Z.q.S, Z.SR, P.q.S, P.SR);
@@ -471,8 +490,23 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
         if (j + 2 > je) break;
         row(j + 2, s2, s0, s1);
     }
+}
+
+// SURR: 0 exact thermodynamics, 1 surrogate, 2 surrogate with split
+// activations. At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
+// the exact thermodynamics: the kernel is latency-bound below that. The
+// surrogate variants hold the MFMA operands and run at 2.
+template <bool IDENT, int SURR>
+__global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
+    __shared__ StepLds<SURR> L;
+    load_step_lds<SURR>(L, a.mlp, a.trust, threadIdx.y * blockDim.x + threadIdx.x,
+                        blockDim.x * blockDim.y);
+    __syncthreads();
+    unsigned nfb = 0, nall = 0;
+    step_tile<IDENT, SURR>(a, L.lntab, surr_lds<SURR>(L), blockIdx.x,
+                           blockIdx.y * blockDim.y + threadIdx.y, nfb, nall);
     if constexpr (SURR != 0) {
-        if (lane == 0) {
+        if (threadIdx.x == 0 && nall > 0) {
             atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
             atomicAdd(&a.fb_count[1], (unsigned long long)nall);
         }
@@ -501,6 +535,142 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
             hipLaunchKernelGGL((step_mixed_kernel<false, 0>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched ensemble
+// ---------------------------------------------------------------------------
+// Vp_ramp (fun.cu:1974-1981) as the host evaluates it
+__device__ __forceinline__ double batch_vp(const BatchRun& R, double t) {
+    if (R.ramp) {
+        t = t * 0.001;
+        return R.Vp + R.Vp_rate * t;
+    }
+    return R.Vp;
+}
+
+__device__ __forceinline__ int batch_run_of(const int* pre, int nruns, int q) {
+    int r = 0;
+    while (r + 1 < nruns && q >= pre[r + 1]) ++r;
+    return r;
+}
+
+// Persistent: each workgroup loads the LDS tables once and walks the work
+// list (run, column tile, band group) with a fixed stride, so the assignment
+// and the results do not depend on timing.
+template <bool IDENT, int SURR>
+__global__ __launch_bounds__(256, SURR ? 2 : 4) void batch_step_kernel(BatchLaunch L) {
+    __shared__ StepLds<SURR> lds;
+    load_step_lds<SURR>(lds, L.runs[0].a.mlp, L.runs[0].a.trust,
+                        threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
+    __syncthreads();
+    for (int item = blockIdx.x; item < L.items; item += gridDim.x) {
+        const int r = batch_run_of(L.item0, L.nruns, item);
+        const BatchRun& R = L.runs[r];
+        const BatchDyn d = L.dyn[r];
+        StepArgsMixed a = R.a;
+        a.psi = R.psi[d.cur];
+        a.c = R.c[d.cur];
+        a.psi_n = R.psi[d.cur ^ 1];
+        a.c_n = R.c[d.cur ^ 1];
+        a.T32 = R.T32[d.tcur];
+        a.ile = d.ile_interp;
+        if (!R.tfc) {
+            // frozen T of the current frame (fun.cu:1610-1619)
+            const double dl = R.GG * (d.l_offset_Vp - R.l_off0) * R.S * R.W0 / R.deltaT0;
+            a.fta_a = R.Tmelt + R.deltaT0 * (R.delta_initial - dl) + R.gstep * R.ioff - kMixedTref;
+        }
+        const int local = item - L.item0[r];
+        const int tx = local % R.ntx;
+        const int bg = local / R.ntx;
+        unsigned nfb = 0, nall = 0;
+        step_tile<IDENT, SURR>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
+                               nfb, nall);
+        if constexpr (SURR != 0) {
+            if (threadIdx.x == 0 && nall > 0) {
+                atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
+                atomicAdd(&a.fb_count[1], (unsigned long long)nall);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_temperature_kernel(BatchLaunch L) {
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < L.cells; q += gridDim.x * blockDim.x) {
+        const int r = batch_run_of(L.cell0, L.nruns, q);
+        const BatchRun& R = L.runs[r];
+        const BatchDyn d = L.dyn[r];
+        TempArgs t = R.t;
+        t.T = R.T[d.tcur];
+        t.Tn = R.T[d.tcur ^ 1];
+        t.T32n = R.T32[d.tcur ^ 1];
+        t.vadv = batch_vp(R, d.step * R.tau0 * R.dt) * R.tau0 / R.SW0;
+        const int local = q - L.cell0[r];
+        temperature_cell(t, local % t.NxT, local / t.NxT);
+    }
+}
+
+// host-loop state update of fun.cu:498-509 for every run
+__global__ void batch_advance_kernel(BatchLaunch L) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= L.nruns) return;
+    const BatchRun& R = L.runs[r];
+    BatchDyn d = L.dyn[r];
+    if (R.tfc) {
+        d.tcur ^= 1;
+        d.ile_interp = R.t.ile;
+    }
+    d.cur ^= 1;
+    d.l_offset_Vp += batch_vp(R, d.step * R.tau0 * R.dt) * R.dt * R.tau0 / R.SW0;
+    d.step += 1;
+    L.dyn[r] = d;
+}
+
+template <bool IDENT, int SURR>
+static hipError_t batch_step_t(const BatchLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL((batch_step_kernel<IDENT, SURR>), dim3(L.step_blocks), dim3(kWave, L.wpb),
+                       0, s, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
+    hipError_t e;
+    switch (L.surr * 2 + (L.ident ? 1 : 0)) {
+    case 0: e = batch_step_t<false, 0>(L, s); break;
+    case 1: e = batch_step_t<true, 0>(L, s); break;
+    case 2: e = batch_step_t<false, 1>(L, s); break;
+    case 3: e = batch_step_t<true, 1>(L, s); break;
+    case 4: e = batch_step_t<false, 2>(L, s); break;
+    default: e = batch_step_t<true, 2>(L, s); break;
+    }
+    if (e != hipSuccess) return e;
+    if (L.cells > 0) {
+        hipLaunchKernelGGL(batch_temperature_kernel, dim3(L.T_blocks), dim3(256), 0, s, L);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(batch_advance_kernel, dim3((L.nruns + 63) / 64), dim3(64), 0, s, L);
+    return hipGetLastError();
+}
+
+template <bool IDENT, int SURR>
+static int resident_t(int wpb) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step_kernel<IDENT, SURR>,
+                                                     kWave * wpb, 0) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+int batch_step_resident(const BatchLaunch& L) {
+    switch (L.surr * 2 + (L.ident ? 1 : 0)) {
+    case 0: return resident_t<false, 0>(L.wpb);
+    case 1: return resident_t<true, 0>(L.wpb);
+    case 2: return resident_t<false, 1>(L.wpb);
+    case 3: return resident_t<true, 1>(L.wpb);
+    case 4: return resident_t<false, 2>(L.wpb);
+    default: return resident_t<true, 2>(L.wpb);
+    }
 }
 
 void sgte_table_info(double* T_lo, double* T_hi, double* max_err) {

@@ -56,6 +56,47 @@ hipError_t launch_pull_back_f32(const float* psi, const double* c, float* psi_n,
 hipError_t launch_tip_column_f32(const float* psi, int Nx, int Ny, float pin, int* rowbuf,
                                  int* out, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// batched ensemble: many independent runs stepped by one persistent launch
+// ---------------------------------------------------------------------------
+// Per-run descriptor, fixed between pull-backs. The field pointers of the
+// step arguments are chosen on the device from the run's parity.
+struct BatchRun {
+    StepArgsMixed a;
+    float* psi[2];
+    double* c[2];
+    float* T32[2];
+    double* T[2];
+    TempArgs t;       // coarse-T arguments; T/Tn/T32n and vadv set on the device
+    int ntx, nbg;     // column tiles, band groups (bands / waves per block)
+    int tfc, ncT;     // coupled T; coarse cells updated per step (NxT * NyT)
+    // time-dependent scalars, restated from the host loop (fun.cu:508-509,
+    // 1610-1619, 1974-1981) with the same FP64 operations
+    double Vp, Vp_rate, tau0, dt, S, W0, SW0;
+    int ramp, ioff;
+    double Tmelt, deltaT0, delta_initial, GG, l_off0, gstep;
+};
+// Per-run state advanced on the device by the captured step sequence
+struct BatchDyn {
+    long long step;
+    double l_offset_Vp;
+    int cur, tcur, ile_interp, pad;
+};
+struct BatchLaunch {
+    const BatchRun* runs;
+    BatchDyn* dyn;
+    const int* item0;   // nruns + 1 prefix sums of step work items
+    const int* cell0;   // nruns + 1 prefix sums of coarse cells
+    int nruns, items, cells;
+    int step_blocks, wpb, T_blocks;
+    bool ident;
+    int surr;           // 0 exact, 1 surrogate, 2 split-activation surrogate
+};
+// one step of every run: fused step kernel, coarse T kernel, state advance
+hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);
+// blocks of the step kernel resident per CU for waves-per-block wpb
+int batch_step_resident(const BatchLaunch& L);
+
 // facts of the build-time SGTE table: window [T_lo, T_hi] (K) and the
 // largest FP32-evaluation error against the FP64 routine (J/mol)
 void sgte_table_info(double* T_lo, double* T_hi, double* max_err);

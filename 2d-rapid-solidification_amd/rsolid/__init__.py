@@ -69,6 +69,14 @@ def _load() -> C.CDLL:
                                         P(C.c_double)]),
         "rs_surrogate_eval": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_double)]),
         "rs_surrogate_stats": (C.c_int, [vp, P(C.c_double), C.c_int32]),
+        "rs_batch_create": (C.c_int, [P(RsParams), C.c_int32, P(vp)]),
+        "rs_batch_destroy": (C.c_int, [vp]),
+        "rs_batch_last_error": (C.c_char_p, [vp]),
+        "rs_batch_run": (C.c_int, [vp, C.c_int32, P(vp)]),
+        "rs_batch_step": (C.c_int, [vp, C.c_int64]),
+        "rs_batch_surrogate_train": (C.c_int, [vp, P(RsSurrogateOpts), P(C.c_double), C.c_int32]),
+        "rs_batch_tune": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int32]),
+        "rs_batch_info": (C.c_int, [vp, P(C.c_double), C.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -97,6 +105,18 @@ def config_params(N: int, tfc: bool = False, alpha_ratio: float = 100.0,
     if rc:
         raise RsError(rc, "rs_params_config")
     p.precision = precision
+    return p
+
+
+def ensemble_params(N: int, deltaT: float, Vp: float, precision: int = RS_PREC_MIXED,
+                    tfc: bool = True) -> RsParams:
+    """Ti-10at%Nb configuration on an N x N grid with undercooling deltaT (K)
+    and pulling speed Vp (m/s): the coarse T sample is shifted so the initial
+    front sits deltaT below the liquidus, as in the preset."""
+    p = config_params(N, tfc, precision=precision)
+    p.TCOLD += p.deltaT0 - deltaT
+    p.deltaT0 = deltaT
+    p.Vp = Vp
     return p
 
 
@@ -295,3 +315,85 @@ class Context:
         n = C.c_int64()
         self._chk(lib.rs_timing(self._h, which, C.byref(ms), C.byref(n)), "rs_timing")
         return ms.value, n.value
+
+
+class _RunView(Context):
+    """A run of a Batch: the Context interface on a context the batch owns."""
+
+    def __init__(self, h, p: RsParams):  # noqa: D401  (no rs_create)
+        self.p = p.copy()
+        self._h = h
+        self.k0 = derive(self.p)
+
+    def close(self) -> None:  # owned by the batch
+        self._h = None
+
+
+class Batch:
+    """Many independent runs stepped by one persistent launch per step,
+    replayed from a captured HIP graph (rs_batch_* in include/rsolid.h)."""
+
+    def __init__(self, params: list):
+        arr = (RsParams * len(params))()
+        for q, p in enumerate(params):
+            C.memmove(C.byref(arr[q]), C.byref(p), C.sizeof(RsParams))
+        h = C.c_void_p()
+        rc = lib.rs_batch_create(arr, len(params), C.byref(h))
+        if rc:
+            raise RsError(rc, "rs_batch_create")
+        self._h = h
+        self.runs = []
+        for q, p in enumerate(params):
+            c = C.c_void_p()
+            self._chk(lib.rs_batch_run(self._h, q, C.byref(c)), "rs_batch_run")
+            self.runs.append(_RunView(c, p))
+
+    def _chk(self, rc: int, what: str) -> None:
+        if rc:
+            raise RsError(rc, f"{what}: {lib.rs_batch_last_error(self._h).decode()}")
+
+    def close(self) -> None:
+        if self._h:
+            for r in self.runs:
+                r._h = None
+            lib.rs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, n: int = 1) -> None:
+        self._chk(lib.rs_batch_step(self._h, n), "rs_batch_step")
+
+    def surrogate_train(self, opts: RsSurrogateOpts | None = None, **kw) -> dict:
+        o = opts if opts is not None else surrogate_defaults(**kw)
+        rep = (C.c_double * RS_SURR_NREPORT)()
+        self._chk(lib.rs_batch_surrogate_train(self._h, C.byref(o), rep, RS_SURR_NREPORT),
+                  "rs_batch_surrogate_train")
+        return dict(zip(SURR_REPORT_KEYS, list(rep)))
+
+    def tune(self, steps: int = 10) -> dict:
+        v = (C.c_double * 32)()
+        self._chk(lib.rs_batch_tune(self._h, steps, v, 32), "rs_batch_tune")
+        grids = {}
+        q = 2
+        while q + 1 < 32 and v[q] > 0:
+            grids[int(v[q])] = int(v[q + 1])
+            q += 2
+        return {"ms_per_step": v[0], "grid_mult": int(v[1]), "band_h": grids}
+
+    def info(self) -> dict:
+        v = (C.c_double * 9)()
+        self._chk(lib.rs_batch_info(self._h, v, 9), "rs_batch_info")
+        keys = ("runs", "cells", "items", "blocks", "waves_per_block", "graph_steps", "last_ms",
+                "last_steps", "grid_mult")
+        return dict(zip(keys, list(v)))

@@ -15,7 +15,9 @@ from the same initial state: `value` is the headline path's MLUPS, `paths`
 holds all three, and `tip_velocity_err` compares each path's tip advance over
 the timed steps with the plain FP64 path's. The surrogate is trained on the
 GPU before the warm-up (outside the timed region; its time is reported).
-Prints ONE JSON line.
+`ensemble` adds the batched-ensemble measurement (64 runs of uneven sizes
+and undercoolings, one persistent launch per step, HIP-graph replay,
+autotuned tiles; see run_ensemble). Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -146,6 +148,92 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     }
 
 
+def run_ensemble(nruns: int, steps: int, warmup: int, seed: int, tune_steps: int,
+                 reference: bool = True, surrogate: bool = True):
+    """Batched ensemble: nruns Ti-10at%Nb runs, grid sizes drawn by seed from
+    {256^2, 512^2, 1024^2, 2048^2} (probabilities 0.4/0.3/0.2/0.1, so the
+    counts are uneven), undercooling ~ U[25, 60] K and pulling speed
+    ~ U[0.03, 0.3] m/s, FP32 fields with one shared CALPHAD surrogate, one
+    persistent step launch per step replayed from a HIP graph, tile shapes
+    from the autotuner. Aggregate MLUPS = sum(Nx*Ny) * steps / wall time;
+    each run's tip and front advance over the timed steps is compared with
+    the same run stepped alone on the plain FP64 path."""
+    import numpy as np
+    import rsolid
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([256, 512, 1024, 2048], size=nruns, p=[0.4, 0.3, 0.2, 0.1])
+    dts = rng.uniform(25.0, 60.0, nruns)
+    vps = rng.uniform(0.03, 0.3, nruns)
+    seeds = [int(x) for x in rng.integers(1, 2**31, nruns)]
+    prec = rsolid.RS_PREC_SURROGATE if surrogate else rsolid.RS_PREC_MIXED
+    params = [rsolid.ensemble_params(int(n), float(d), float(v), prec)
+              for n, d, v in zip(sizes, dts, vps)]
+    b = rsolid.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+    rep = b.surrogate_train() if surrogate else None
+    tuned = b.tune(steps=tune_steps)
+    b.step(warmup)
+    x0 = [_tip_x(r) for r in b.runs]
+    f0 = [_front_x(r) for r in b.runs]
+    if surrogate:
+        for r in b.runs:
+            r.surrogate_stats(reset=True)
+    t0 = time.perf_counter()
+    b.step(steps)
+    el = time.perf_counter() - t0
+    info = b.info()
+    tip_adv = [_tip_x(r) - a for r, a in zip(b.runs, x0)]
+    front_adv = [_front_x(r) - a for r, a in zip(b.runs, f0)]
+    fb = [r.surrogate_stats() for r in b.runs] if surrogate else None
+    b.close()
+    cells = float(sum(int(n) * int(n) for n in sizes))
+    out = {"runs": nruns,
+           "grid_counts": {f"{n}^2": int((sizes == n).sum()) for n in (256, 512, 1024, 2048)},
+           "cells": cells, "steps": steps, "warmup": warmup,
+           "value": round(cells * steps / el / 1e6, 2), "unit": "MLUPS",
+           "ms_per_step": round(el * 1e3 / steps, 4),
+           "gpu_ms_per_step": round(info["last_ms"] / max(1, info["last_steps"]), 4),
+           "persistent_blocks": int(info["blocks"]), "work_items": int(info["items"]),
+           "graph_steps": int(info["graph_steps"]), "tuned": tuned,
+           "path": "surrogate" if surrogate else "fp32 exact table",
+           "deltaT_range": [round(float(dts.min()), 2), round(float(dts.max()), 2)],
+           "Vp_range": [round(float(vps.min()), 4), round(float(vps.max()), 4)]}
+    if surrogate:
+        out["surrogate_trusted_bins"] = rep["trusted_frac"]
+        out["surrogate_window_K"] = [round(rep["T_lo"], 2), round(rep["T_hi"], 2)]
+        out["fallback_frac"] = (sum(f["fallback"] for f in fb)
+                                / max(1.0, sum(f["evaluated"] for f in fb)))
+    if reference:
+        terr, ferr = [], []
+        for p, s, ta, fa in zip(params, seeds, tip_adv, front_adv):
+            q = p.copy()
+            q.precision = rsolid.RS_PREC_FP64
+            c = rsolid.Context(q)
+            c.init_config(s)
+            c.step(warmup)
+            a, f = _tip_x(c), _front_x(c)
+            c.step(steps)
+            tr, fr = _tip_x(c) - a, _front_x(c) - f
+            c.close()
+            if abs(tr) > 1e-9:
+                terr.append(abs(ta - tr) / abs(tr))
+            if abs(fr) > 1e-9:
+                ferr.append(abs(fa - fr) / abs(fr))
+        out["tip_velocity_err"] = {"max": round(max(terr), 5) if terr else None,
+                                   "median": round(float(np.median(terr)), 6) if terr else None}
+        out["front_velocity_err"] = {"max": round(max(ferr), 5) if ferr else None,
+                                     "median": round(float(np.median(ferr)), 6) if ferr else None}
+    return out
+
+
+def _front_x(ctx) -> float:
+    """Mean interface position over the rows (cells, lab frame)."""
+    from rsolid.report import _front_x as fx
+    from rsolid.abi import RS_FIELD_PSI
+    return ctx.constants().i_offset_history + fx(ctx.download(RS_FIELD_PSI))
+
+
 def _tip_x(ctx) -> float:
     """Tip position (cells, lab frame) from the quartic/parabola fit."""
     t = ctx.tip()
@@ -163,6 +251,9 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=None,
                     help="surrogate training epochs (default: the library's); profiler runs "
                          "that only need the step kernel's counters use 1")
+    ap.add_argument("--ensemble-runs", type=int, default=64,
+                    help="runs of the batched-ensemble measurement (0: skip it)")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -229,6 +320,10 @@ def main() -> int:
                             "train_steps": int(r["train"]["steps"]),
                             "trusted_bin_frac": r["train"]["trusted_frac"],
                             "max_residual_trusted": r["train"]["max_res_trusted"]}
+    if a.ensemble_runs > 0 and not a.only:
+        out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5)
+        out["ensemble_fp32_exact_table"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024,
+                                                        5, surrogate=False)
     if head["copy_ms"] is not None:
         nb = 2 * 8 * a.grid * a.grid
         out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(head["copy_ms"], 5),

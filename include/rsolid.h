@@ -306,6 +306,39 @@ int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out);
  * ratio. reset != 0 clears the counters afterwards. */
 int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset);
 
+/* ---- batched ensemble ----------------------------------------------------
+ * No counterpart in the reference (one run per job). Many independent runs
+ * -- any grid sizes, undercoolings and pulling speeds, one precision path
+ * (RS_PREC_MIXED or RS_PREC_SURROGATE) -- advance together: each step is one
+ * persistent launch of the fused step kernel over the work list of all runs
+ * (run, column tile, band group), one coarse-T launch and a state-advance
+ * launch, and the step sequence is captured once in a HIP graph and
+ * replayed. Pull-backs (fun.cu:448) split the replay into segments and run
+ * on the host between them. Results are bitwise those of the runs stepped
+ * one by one with rs_step. Coupled-T runs need ratio_dt_dtT == 1. */
+typedef struct rs_batch rs_batch;
+
+int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out);
+int rs_batch_destroy(rs_batch* b);
+const char* rs_batch_last_error(const rs_batch* b);   /* never NULL */
+/* The context of run i (owned by the batch): init, upload/download, tip,
+ * mass, constants as for a stand-alone context. */
+int rs_batch_run(rs_batch* b, int32_t i, rs_ctx** ctx);
+int rs_batch_step(rs_batch* b, int64_t nsteps);
+/* Trains one surrogate on the union of the runs' temperature windows and
+ * installs it in every run (report as rs_surrogate_train). */
+int rs_batch_surrogate_train(rs_batch* b, const rs_surrogate_opts* o, double* report, int32_t n);
+/* Autotuner: band height per grid (from 10, 20, 30, 40, 60, 80 rows) and the
+ * persistent grid size (1, 2, 4 x resident blocks), each timed over `steps`
+ * graph replays on a snapshot of the state, which is restored. out: [0] best
+ * ms per step, [1] grid multiplier, then (Nx, band height) per distinct
+ * grid. Choices are cached per (precision, grid) for later batches. */
+int rs_batch_tune(rs_batch* b, int32_t steps, double* out, int32_t n);
+/* out: [0] runs, [1] cells (sum of Nx*Ny), [2] work items, [3] persistent
+ * blocks, [4] waves per block, [5] steps per captured graph, [6] ms of the
+ * last rs_batch_step (HIP events), [7] its steps, [8] grid multiplier. */
+int rs_batch_info(rs_batch* b, double* out, int32_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,104 @@
+"""Batched ensemble (one persistent step launch per step for all runs, the
+step sequence replayed from a HIP graph, autotuned tile shapes) against the
+same runs stepped one by one with rs_step: the results must be bitwise
+identical, across graph replays, pull-back segments and tuning."""
+import sys
+
+import numpy as np
+import pytest
+
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED, RS_PREC_SURROGATE
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [(256, 43.24, 0.0842), (320, 30.0, 0.05), (512, 60.0, 0.15), (256, 25.0, 0.2)]
+
+
+def say(*a):
+    print("[gpu-batch]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _fields(ctx):
+    return [ctx.download(f) for f in (RS_FIELD_PSI, RS_FIELD_C, RS_FIELD_T)]
+
+
+def _singles(rs, params, seeds, nsteps, pull_back=None, surrogate=None):
+    out = []
+    for p, s in zip(params, seeds):
+        c = rs.Context(p)
+        c.init_config(s)
+        if pull_back:
+            k = c.constants()
+            k.step_pull_back = pull_back
+            c.set_constants(k)
+        if surrogate is not None:
+            c.surrogate_set(*surrogate)
+        c.step(nsteps)
+        out.append((_fields(c), c.constants().step, c.constants().i_offset_history))
+        c.close()
+    return out
+
+
+def _check(batch, singles):
+    for q, (run, (ref, step, ioh)) in enumerate(zip(batch.runs, singles)):
+        got = _fields(run)
+        k = run.constants()
+        assert k.step == step and k.i_offset_history == ioh, q
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), q
+
+
+@pytest.mark.parametrize("pull_back", [None, 150])
+def test_batch_matches_single_runs(rs, pull_back):
+    params = [rs.ensemble_params(n, dT, v, RS_PREC_MIXED) for n, dT, v in SPECS]
+    seeds = [11, 12, 13, 14]
+    say("batch of", len(params), "pull_back", pull_back)
+    b = rs.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+        if pull_back:
+            k = run.constants()
+            k.step_pull_back = pull_back
+            run.set_constants(k)
+    b.step(7)          # below one captured sequence
+    b.step(293)        # graph replays, a pull-back segment boundary when set
+    say("info", b.info())
+    _check(b, _singles(rs, params, seeds, 300, pull_back))
+    b.close()
+
+
+def test_batch_tune_keeps_state_and_results(rs):
+    params = [rs.ensemble_params(n, dT, v, RS_PREC_MIXED) for n, dT, v in SPECS]
+    seeds = [21, 22, 23, 24]
+    b = rs.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+    b.step(20)
+    before = [_fields(r) for r in b.runs]
+    say("tuning")
+    t = b.tune(steps=5)
+    say("tuned", t)
+    for r, f in zip(b.runs, before):
+        assert r.constants().step == 20
+        for a, c in zip(_fields(r), f):
+            assert np.array_equal(a, c)
+    b.step(180)
+    _check(b, _singles(rs, params, seeds, 200))
+    b.close()
+
+
+def test_batch_surrogate_matches_single_runs(rs):
+    params = [rs.ensemble_params(n, dT, v, RS_PREC_SURROGATE) for n, dT, v in SPECS[:3]]
+    seeds = [31, 32, 33]
+    b = rs.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+    rep = b.surrogate_train(epochs=4)
+    say("shared surrogate", rep)
+    weights = b.runs[0].surrogate_get()
+    b.step(150)
+    fb = [r.surrogate_stats()["fallback_frac"] for r in b.runs]
+    say("fallback per run", fb)
+    _check(b, _singles(rs, params, seeds, 150, surrogate=weights))
+    b.close()
