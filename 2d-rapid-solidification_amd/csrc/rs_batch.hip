@@ -323,6 +323,7 @@ int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out) {
         if (params[r].precision != RS_PREC_MIXED && params[r].precision != RS_PREC_SURROGATE)
             return RS_ERR_ARG;  // the batched kernels are the FP32-field kernels
         if (params[r].precision != params[0].precision) return RS_ERR_ARG;
+        if (params[r].at_coef != 0.0) return RS_ERR_ARG;  // no antitrapping pass in the batch
     }
     rs_batch* b = new (std::nothrow) rs_batch;
     if (!b) return RS_ERR_NOMEM;

@@ -46,6 +46,10 @@ int check_params(const rs_params& p, std::string* why) {
         return RS_ERR_ARG;
     }
     if (p.ifTFC && p.Ny < 22) { *why = "coupled T needs Ny >= 22"; return RS_ERR_ARG; }
+    if (p.at_coef != 0.0 && !(p.k_part > 0.0 && p.V_D >= 0.0)) {
+        *why = "antitrapping needs 0 < k_part and V_D >= 0";
+        return RS_ERR_ARG;
+    }
     if (p.i_tip_target < 1 || p.i_tip_target > p.Nx - 2) { *why = "i_tip_target outside the grid"; return RS_ERR_ARG; }
     if ((int64_t)p.Nx * p.Ny >= (int64_t)1 << 31) { *why = "grid exceeds 2^31 cells (32-bit offsets)"; return RS_ERR_ARG; }
     if (p.precision != RS_PREC_FP64) {
@@ -583,6 +587,18 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
         } else {
             const StepArgsF64 a = step_args(ctx);
             HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+        }
+        if (p.at_coef != 0.0) {
+            const double vdn = p.V_D > 0.0 ? p.V_D * tau0 / (p.S * p.W0) : 0.0;
+            const int c0 = ctx->cur;
+            if (ctx->mixed)
+                HIPCHK(ctx, launch_antitrap_f32(ctx->psi32[c0], ctx->psi32[c0 ^ 1], ctx->c[c0],
+                                                ctx->c[c0 ^ 1], p.Nx, p.Ny, p.dx, k.dt, p.at_coef,
+                                                p.k_part, vdn, ctx->stream));
+            else
+                HIPCHK(ctx, launch_antitrap(ctx->psi[c0], ctx->psi[c0 ^ 1], ctx->c[c0], ctx->c[c0 ^ 1],
+                                            p.Nx, p.Ny, p.dx, k.dt, p.at_coef, p.k_part, vdn,
+                                            ctx->stream));
         }
         if (ctx->timing) {
             int rc = ev_record(ctx, 0);

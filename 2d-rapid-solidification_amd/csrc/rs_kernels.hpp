@@ -115,6 +115,14 @@ hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* 
                        hipStream_t s);
 hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
 hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s);
+// antitrapping + solute-trapping correction of c_n (extension; psi in FP64
+// or FP32); vdn = V_D in W/tau0 units (0: no trapping factor)
+hipError_t launch_antitrap(const double* psi, const double* psi_n, const double* c, double* c_n,
+                           int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
+                           hipStream_t s);
+hipError_t launch_antitrap_f32(const float* psi, const float* psi_n, const double* c, double* c_n,
+                               int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
+                               hipStream_t s);
 
 // host-side helpers that fill the temperature source from the run state
 TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tcoarse, int ile);

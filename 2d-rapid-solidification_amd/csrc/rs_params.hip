@@ -119,6 +119,9 @@ extern "C" int rs_params_reference(rs_params* p, int32_t Nx, int32_t Ny) {
     p->phi_interface = 0.0;              // fun.cu:126
     p->c_liq0 = p->cinf;
     p->c_sol0 = 0.515285917;             // fun.cu:62 comment (solid partner of cinf)
+    p->at_coef = 0.0;                    // no antitrapping in the reference
+    p->V_D = 0.0;
+    p->k_part = p->c_sol0 / p->cinf;
     return RS_OK;
 }
 
@@ -173,6 +176,9 @@ extern "C" int rs_params_config(rs_params* p, int32_t N, int32_t tfc, double alp
     if (liquidus(*p, p->c_liq0, &TL, &cs) != 0) return RS_ERR_DOMAIN;
     p->Tmelt = TL;
     p->c_sol0 = cs;
+    p->at_coef = 0.0;                   // antitrapping / trapping extension off
+    p->V_D = 0.0;
+    p->k_part = cs / p->c_liq0;
     p->cinf = p->c_liq0;
     p->deltaT0 = 2161.59 - 2118.35;
     // coarse T sample twice the window, window centred in it

@@ -45,6 +45,7 @@ class RsParams(C.Structure):
         ("L1_liq", _D), ("dt_coef", _D), ("dtT_coef", _D), ("length_sample_um", _D),
         ("lap_w_axis", _D), ("lap_w_diag", _D), ("q_solid", _D), ("total_time", _D),
         ("delta_initial", _D), ("phi_interface", _D), ("c_liq0", _D), ("c_sol0", _D),
+        ("at_coef", _D), ("V_D", _D), ("k_part", _D),
     ]
 
     def as_dict(self) -> dict:

@@ -118,6 +118,15 @@ typedef struct rs_params {
     double delta_initial;      /* -1 (fun.cu:2013)                            */
     double phi_interface;      /* 0 (fun.cu:126)                              */
     double c_liq0, c_sol0;     /* rs_init_config compositions (Ti fraction)   */
+    /* Extension absent from the reference, off when at_coef == 0: the
+     * antitrapping solute current J = a_t W (1 - k) c_L (dphi/dt) n
+     * (n: unit normal into the liquid; Karma, PRL 87 (2001) 115701) times the
+     * solute-trapping factor 1/(1 + |V|/V_D), V = (dphi/dt)/|grad phi| the
+     * local interface speed, which turns the partition into Aziz's
+     * continuous-growth k(V) = (k + V/V_D)/(1 + V/V_D). */
+    double at_coef;            /* a_t; 1/(2 sqrt 2) for phi in [-1, 1]       */
+    double V_D;                /* diffusive speed, m/s; 0 = no trapping term */
+    double k_part;             /* equilibrium partition c_S / c_L of c       */
 } rs_params;
 
 /* Run state: field-for-field the reference's struct constants

@@ -57,6 +57,7 @@ def _load() -> C.CDLL:
                                       P(d)]),
         "or_tip": (C.c_int, [vp, P(RsTipRow)]),
         "or_common_tangent": (C.c_int, [P(RsParams), d, P(d), P(d)]),
+        "or_at_flux": (d, [P(RsParams), d, d, d, d, d, d, d, d, d, d, d, d]),
         "or_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
         "or_set_threads": (C.c_int, [C.c_int]),
     }
@@ -120,6 +121,10 @@ def anisotropy(p: RsParams, ph3x3: np.ndarray, orient=None):
     a_k = C.c_double()
     v = lib.or_anisotropy(_P(p), _dptr(ph), _dptr(o), C.byref(a_s), C.byref(a_k))
     return v, a_s.value, a_k.value
+
+
+def at_flux(p, vdn, dt, pP, pQ, nP, nQ, cP, cQ, t1=0.0, t2=0.0, t3=0.0, t4=0.0) -> float:
+    return lib.or_at_flux(_P(p), vdn, dt, pP, pQ, nP, nQ, cP, cQ, t1, t2, t3, t4)
 
 
 def common_tangent(p: RsParams, T: float, cl0: float, cs0: float):

@@ -604,6 +604,70 @@ static void or_compute_c(or_state* s) {
     or_ghosts(s, s->c_next);
 }
 
+/* Antitrapping current with the solute-trapping factor (an extension the
+ * reference does not have; rs_params.at_coef/V_D/k_part). Face flux between
+ * cells P and Q (Q = P + unit step along the face normal):
+ *   J = a_t (1-k) c_L dphidt (-g_n/|g|) / (1 + |V|/V_D),
+ *   g_n = (phi_Q - phi_P)/dx, g_t from the four cells beside the face,
+ *   dphidt = mean over P, Q of (phi_next - phi)/dt, V = dphidt/|g|,
+ *   c_L = c_f / (1 - h_f (1-k)), c_f, h_f = (1+phi_f)/2 face means,
+ * and c_next -= dt (div J), then the clamp at 1e-6 and the ghost copies. */
+double or_at_flux(const rs_params* p, double vdn, double dt, double pP, double pQ, double nP,
+                  double nQ, double cP, double cQ, double t1, double t2, double t3, double t4) {
+    const double dx = p->dx;
+    const double gn = (pQ - pP) / dx;
+    const double gt = ((t1 + t2) - (t3 + t4)) / (4.0 * dx);
+    const double g2 = gn * gn + gt * gt;
+    if (!(g2 > 1e-20)) return 0.0;
+    const double g = sqrt(g2);
+    const double dpt = 0.5 * ((nP - pP) + (nQ - pQ)) / dt;
+    double trap = 1.0;
+    if (vdn > 0.0) trap = 1.0 / (1.0 + fabs(dpt / g) / vdn);
+    const double k = p->k_part;
+    const double cf = 0.5 * (cP + cQ);
+    const double hf = 0.5 * (1.0 + 0.5 * (pP + pQ));
+    const double cl = cf / (1.0 - hf * (1.0 - k));
+    return p->at_coef * (1.0 - k) * cl * dpt * (-gn / g) * trap;
+}
+
+static void or_antitrap(or_state* s) {
+    const rs_params* p = &s->p;
+    const int Nx = p->Nx, Ny = p->Ny;
+    const double dt = s->k.dt;
+    const double tau0 = or_tau0(p);
+    const double vdn = p->V_D > 0.0 ? p->V_D * tau0 / (p->S * p->W0) : 0.0;
+    const double* f = s->phi;
+    const double* g = s->phi_next;
+    const double* c = s->c;
+#define F(i, j) f[POS(s, (i), (j))]
+#define G(i, j) g[POS(s, (i), (j))]
+#define CC(i, j) c[POS(s, (i), (j))]
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < Ny - 1; j++) {
+        for (int i = 1; i < Nx - 1; i++) {
+            /* x faces (i-1/2, j) and (i+1/2, j), y faces (i, j-1/2), (i, j+1/2) */
+            const double Jxm = or_at_flux(p, vdn, dt, F(i - 1, j), F(i, j), G(i - 1, j), G(i, j),
+                                          CC(i - 1, j), CC(i, j), F(i - 1, j + 1), F(i, j + 1),
+                                          F(i - 1, j - 1), F(i, j - 1));
+            const double Jxp = or_at_flux(p, vdn, dt, F(i, j), F(i + 1, j), G(i, j), G(i + 1, j),
+                                          CC(i, j), CC(i + 1, j), F(i, j + 1), F(i + 1, j + 1),
+                                          F(i, j - 1), F(i + 1, j - 1));
+            const double Jym = or_at_flux(p, vdn, dt, F(i, j - 1), F(i, j), G(i, j - 1), G(i, j),
+                                          CC(i, j - 1), CC(i, j), F(i + 1, j - 1), F(i + 1, j),
+                                          F(i - 1, j - 1), F(i - 1, j));
+            const double Jyp = or_at_flux(p, vdn, dt, F(i, j), F(i, j + 1), G(i, j), G(i, j + 1),
+                                          CC(i, j), CC(i, j + 1), F(i + 1, j), F(i + 1, j + 1),
+                                          F(i - 1, j), F(i - 1, j + 1));
+            const double dc = -dt * ((Jxp - Jxm) + (Jyp - Jym)) / p->dx;
+            s->c_next[POS(s, i, j)] = fmax(s->c_next[POS(s, i, j)] + dc, 1.0e-6);
+        }
+    }
+#undef F
+#undef G
+#undef CC
+    or_ghosts(s, s->c_next);
+}
+
 /* integrate_phi (fun.cu:1494-1535) */
 static void or_integrate_phi(or_state* s, int i_left_end) {
     const int Nx = s->p.Nx, r = s->p.ratio_dxT_dx;
@@ -715,6 +779,7 @@ int or_step(or_state* s, int64_t nsteps) {
         or_compute_phi(s);
         or_compute_psi(s);
         or_compute_c(s);
+        if (p->at_coef != 0.0) or_antitrap(s);
         if (p->ifTFC) {
             or_integrate_phi(s, s->k.i_left_end);
             for (int q = 0; q < s->k.ratio_dt_dtT; q++) or_compute_temperature(s, step);

@@ -79,6 +79,11 @@ int or_tip(or_state* s, rs_tip_row* tip);
 /* Common tangent of the liquid and bcc Gibbs energies at temperature T:
  * returns 0 and writes (c_liquid, c_solid) on convergence. */
 int or_common_tangent(const rs_params* p, double T, double* cl, double* cs);
+/* antitrapping face flux of the trapping extension (rs_params.at_coef):
+ * cells P, Q along the face normal, old/new phi, old c, the four tangential
+ * neighbours (t1 + t2) - (t3 + t4) across the face; vdn = V_D in W/tau0 */
+double or_at_flux(const rs_params* p, double vdn, double dt, double pP, double pQ, double nP,
+                  double nQ, double cP, double cQ, double t1, double t2, double t3, double t4);
 
 /* splitmix64 stream used by the seeded configuration initialiser. */
 uint64_t or_splitmix64(uint64_t* state);

@@ -112,3 +112,31 @@ def test_mixed_full_size_determinism(rs):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert np.isfinite(outs[0][0]).all() and np.isfinite(outs[0][1]).all()
+
+
+def test_mixed_antitrapping_tracks_oracle(rs, oracle_mod):
+    """The antitrapping / trapping extension on the mixed path (FP32 psi,
+    FP64 correction of c) against the FP64 oracle, same tolerances as the
+    plain mixed run."""
+    pm = rs.config_params(256, True, precision=RS_PREC_MIXED)
+    for p in (pm,):
+        p.at_coef = 1.0 / (2.0 * np.sqrt(2.0))
+        p.V_D = p.Vp / 0.5
+    pd = pm.copy()
+    pd.precision = 0
+    say("mixed antitrapping vs oracle")
+    cm = rs.Context(pm)
+    cm.init_config(7)
+    st = oracle_mod.State(pd)
+    st.init_config(7)
+    m0, mo0 = cm.solute_mass(), st.solute_mass()
+    cm.step(2000)
+    st.step(2000)
+    epsi = float(np.max(np.abs(cm.download(RS_FIELD_PSI) - st.get(RS_FIELD_PSI))))
+    ec = _rel(cm.download(RS_FIELD_C), st.get(RS_FIELD_C))
+    dm = (cm.solute_mass() - m0) / m0 - (st.solute_mass() - mo0) / mo0
+    say("psi Linf", epsi, "c rel", ec, "mass-drift diff", dm)
+    assert epsi <= 2e-6
+    assert ec <= 1e-7
+    assert abs(dm) <= 1e-10
+    cm.close()

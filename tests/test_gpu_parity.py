@@ -198,3 +198,27 @@ def test_full_size_properties(rs):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert outs[0][2] == outs[1][2]
+
+
+@pytest.mark.parametrize("v_ratio", [0.1, 2.0])
+def test_antitrapping_trapping_run_matches_oracle(rs, oracle_mod, v_ratio):
+    """Antitrapping current with the solute-trapping factor (an extension,
+    off in the reference presets) at V_p / V_D = 0.1 and 2, coupled T,
+    2000 steps: FP64 path within 1e-10 of the oracle, and the solute-mass
+    trajectory too."""
+    p = rs.config_params(256, True)
+    p.at_coef = 1.0 / (2.0 * np.sqrt(2.0))
+    p.V_D = p.Vp / v_ratio
+    say("antitrapping V/V_D", v_ratio)
+    ctx = rs.Context(p)
+    st = oracle_mod.State(p)
+    ctx.init_config(7)
+    st.init_config(7)
+    m0, mo0 = ctx.solute_mass(), st.solute_mass()
+    ctx.step(2000)
+    st.step(2000)
+    _compare(ctx, st, True, f"at{v_ratio}")
+    dm = (ctx.solute_mass() - m0) / m0 - (st.solute_mass() - mo0) / mo0
+    assert abs(dm) <= 1e-12
+    ctx.close()
+    st.close()

@@ -335,3 +335,92 @@ def test_tip_column_brute_force(rs, oracle_mod):
         psi[jj, :L[jj]] = 0.5
     s.put(RS_FIELD_PSI, psi)
     assert s.tip_column() == L.max() - 1
+
+
+# ---------------------------------------------------------------------------
+# antitrapping + solute-trapping extension (absent from the reference):
+# closed-form properties of the face flux and of the whole correction
+# ---------------------------------------------------------------------------
+AT = 1.0 / (2.0 * math.sqrt(2.0))
+
+
+def _at_params(rs, k=0.8):
+    p = rs.config_params(64, False)
+    p.at_coef = AT
+    p.k_part = k
+    return p
+
+
+def test_antitrap_flux_closed_forms(rs, oracle_mod):
+    p = _at_params(rs)
+    dt = 0.1
+    # flat interface, solid (phi = 1) at P, liquid at Q: normal into the
+    # liquid is +x, so a solidifying face (dphi/dt > 0) drives solute +x
+    pP, pQ, rate = 0.3, -0.2, 0.05
+    nP, nQ = pP + rate * dt, pQ + rate * dt
+    c = 0.9
+    J = oracle_mod.at_flux(p, 0.0, dt, pP, pQ, nP, nQ, c, c)
+    hf = 0.5 * (1.0 + 0.5 * (pP + pQ))
+    cl = c / (1.0 - hf * (1.0 - p.k_part))
+    assert J == pytest.approx(AT * (1.0 - p.k_part) * cl * rate, rel=1e-14)
+    assert J > 0.0
+    # mirror image: the same face seen from the other side carries -J
+    assert oracle_mod.at_flux(p, 0.0, dt, pQ, pP, nQ, nP, c, c) == pytest.approx(-J, rel=1e-14)
+    # melting reverses it; no interface motion, no current
+    assert oracle_mod.at_flux(p, 0.0, dt, pP, pQ, pP - rate * dt, pQ - rate * dt, c, c) == \
+        pytest.approx(-J, rel=1e-12)
+    assert oracle_mod.at_flux(p, 0.0, dt, pP, pQ, pP, pQ, c, c) == 0.0
+    # no partitioning (k = 1): no current
+    assert oracle_mod.at_flux(_at_params(rs, 1.0), 0.0, dt, pP, pQ, nP, nQ, c, c) == 0.0
+    # c_L limits: all liquid (h = 0) gives c, all solid (h = 1) gives c / k
+    Jl = oracle_mod.at_flux(p, 0.0, dt, -1.0, -1.0 - 1e-3, -1.0 + 1e-3, -1.0, c, c)
+    Js = oracle_mod.at_flux(p, 0.0, dt, 1.0 + 1e-3, 1.0, 1.0 + 2e-3, 1.0 + 1e-3, c, c)
+    assert Jl == pytest.approx(AT * (1 - p.k_part) * c * 1e-3 / dt, rel=2e-3)
+    assert Js == pytest.approx(AT * (1 - p.k_part) * (c / p.k_part) * 1e-3 / dt, rel=2e-3)
+    # a face tangential to the interface (g_n = 0) carries nothing
+    assert oracle_mod.at_flux(p, 0.0, dt, 0.1, 0.1, 0.2, 0.2, c, c, 0.5, 0.5, 0.0, 0.0) == 0.0
+    # an oblique interface: the normal component is g_n / |g|
+    Jo = oracle_mod.at_flux(p, 0.0, dt, pP, pQ, nP, nQ, c, c, 0.4, 0.4, 0.0, 0.0)
+    gn = (pQ - pP) / p.dx
+    gt = 0.8 / (4.0 * p.dx)
+    assert Jo == pytest.approx(J * abs(gn) / math.hypot(gn, gt), rel=1e-13)
+
+
+def test_antitrap_trapping_factor(rs, oracle_mod):
+    p = _at_params(rs)
+    dt, pP, pQ, rate = 0.1, 0.3, -0.2, 0.05
+    nP, nQ = pP + rate * dt, pQ + rate * dt
+    J0 = oracle_mod.at_flux(p, 0.0, dt, pP, pQ, nP, nQ, 0.9, 0.9)
+    V = rate / abs((pQ - pP) / p.dx)            # interface speed dphi/dt / |grad phi|
+    # at V = V_D the current halves: the partition k(V) = (k + 1)/2
+    assert oracle_mod.at_flux(p, V, dt, pP, pQ, nP, nQ, 0.9, 0.9) == pytest.approx(0.5 * J0, rel=1e-13)
+    # V_D -> infinity: no trapping; V_D -> 0: complete trapping
+    assert oracle_mod.at_flux(p, 1e12, dt, pP, pQ, nP, nQ, 0.9, 0.9) == pytest.approx(J0, rel=1e-10)
+    assert abs(oracle_mod.at_flux(p, 1e-12, dt, pP, pQ, nP, nQ, 0.9, 0.9)) < 1e-9 * abs(J0)
+
+
+def test_antitrap_run_conserves_and_moves_solute_to_liquid(rs, oracle_mod):
+    """A growing front (uniform T, so no flux leaves the box): the corrected
+    run conserves solute to rounding, and against the uncorrected run it has
+    moved solute from the solid side to the liquid side."""
+    p = rs.config_params(96, False)
+    p.GG = 0.0
+    p.q_solid = 0.0
+    runs = {}
+    for at in (0.0, AT):
+        q = p.copy()
+        q.at_coef = at
+        st = oracle_mod.State(q)
+        st.init_config(4)
+        m0 = st.solute_mass()
+        st.step(300)
+        runs[at] = (st.get(RS_FIELD_C), st.get(RS_FIELD_PSI), (st.solute_mass() - m0) / m0)
+        st.close()
+    c0, psi0, d0 = runs[0.0]
+    c1, psi1, d1 = runs[AT]
+    assert abs(d1) < 1e-13 and abs(d0) < 1e-13
+    dc = (c1 - c0)[1:-1, 1:-1]
+    x = np.arange(1, p.Nx - 1)[None, :]
+    # the solid is at small i: the correction's first moment points to +x
+    assert float(np.sum(dc * x)) > 0.0
+    assert abs(float(np.sum(dc))) < 1e-10 * float(np.sum(np.abs(dc)))
