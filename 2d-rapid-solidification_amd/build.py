@@ -38,11 +38,14 @@ PER_FILE = {
     "rs_mixed.hip": ["-ffp-contract=off"],
     "rs_surr.hip": ["-ffp-contract=off"],
     "rs_batch.hip": ["-ffp-contract=off"],
+    "rs_io.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
-           "rs_capi.hip", "rs_surr.hip", "rs_batch.hip"]
+           "rs_capi.hip", "rs_surr.hip", "rs_batch.hip", "rs_io.hip"]
 
 LIB = os.path.join(PKG, "rsolid", "librsolid.so")
+DRIVER_SRC = os.path.join(PKG, "driver", "rsolid_run.cpp")
+DRIVER = os.path.join(PKG, "bin", "rsolid_run")
 ORACLE_SRC = os.path.join(ROOT, "oracle", "rs_oracle.c")
 ORACLE_LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 
@@ -82,6 +85,15 @@ def build_hip(jobs: int = 4) -> str:
     return LIB
 
 
+def build_driver() -> str:
+    """The command-line driver (C++, host only) linked against the library."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+          "-L", os.path.dirname(LIB), "-lrsolid", "-Wl,-rpath,$ORIGIN/../rsolid",
+          "-Wl,-rpath-link,/opt/rocm/lib", "-o", DRIVER])
+    return DRIVER
+
+
 def build_oracle() -> str:
     # portable x86-64-v3 code: the library is built here and run on the GPU
     # host's cores as the CPU baseline
@@ -96,6 +108,7 @@ def main() -> None:
     ap.add_argument("--no-oracle", action="store_true")
     a = ap.parse_args()
     print(build_hip(a.jobs))
+    print(build_driver())
     if not a.no_oracle:
         print(build_oracle())
 

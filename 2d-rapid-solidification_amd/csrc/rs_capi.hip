@@ -85,6 +85,8 @@ void free_all(rs_ctx* ctx) {
         if (f) (void)hipFree(f);
     surr_free(ctx);
     if (ctx->rowi) (void)hipFree(ctx->rowi);
+    if (ctx->tipvals) (void)hipFree(ctx->tipvals);
+    if (ctx->tiprow) (void)hipFree(ctx->tiprow);
     if (ctx->outi) (void)hipFree(ctx->outi);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     for (int q = 0; q < 3; ++q)
@@ -645,11 +647,27 @@ int rs_tip(rs_ctx* ctx, rs_tip_row* t) {
     if (!ctx || !t) return RS_ERR_ARG;
     const rs_params& p = ctx->p;
     rs_constants& k = ctx->k;
-    std::vector<double> psi((size_t)ctx->n);
-    int rc = rs_download(ctx, RS_FIELD_PSI, psi.data(), ctx->n);
-    if (rc) return rc;
+    // per-row fit data from the GPU (36 B per row), not the field
+    if (!ctx->tipvals) {
+        HIPCHK(ctx, hipMalloc(&ctx->tipvals, sizeof(double) * 8 * (size_t)p.Ny));
+        HIPCHK(ctx, hipMalloc(&ctx->tiprow, sizeof(int) * (size_t)p.Ny));
+    }
+    if (ctx->mixed)
+        HIPCHK(ctx, launch_tip_detail_f32(ctx->psi32[ctx->cur], p.Nx, p.Ny, p.phi_interface,
+                                          ctx->tiprow, ctx->tipvals, ctx->stream));
+    else
+        HIPCHK(ctx, launch_tip_detail(ctx->psi[ctx->cur], p.Nx, p.Ny, p.phi_interface, ctx->tiprow,
+                                      ctx->tipvals, ctx->stream));
+    std::vector<int> irow((size_t)p.Ny);
+    std::vector<double> vals((size_t)p.Ny * 8);
+    HIPCHK(ctx, hipMemcpyAsync(irow.data(), ctx->tiprow, sizeof(int) * p.Ny, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(vals.data(), ctx->tipvals, sizeof(double) * 8 * p.Ny,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = RS_OK;
     TipFit f;
-    const int st = precise_tip(p, psi.data(), &f);
+    const int st = precise_tip_rows(p, irow.data(), vals.data(), &f);
     const double tau0 = tau0_of(p);
     const double dt = k.dt;
     const double xnow = (k.i_offset_history + f.x) * p.dx;
@@ -662,9 +680,6 @@ int rs_tip(rs_ctx* ctx, rs_tip_row* t) {
         // the reference re-interpolates T_fine with the current i_left_end and
         // samples row 0 (fun.cu:3052-3056)
         k.i_left_end_interp = k.i_left_end;
-        std::vector<double> Tc((size_t)ctx->nT);
-        rc = rs_download(ctx, RS_FIELD_T, Tc.data(), ctx->nT);
-        if (rc) return rc;
         const int r = p.ratio_dxT_dx;
         const int i = f.i, j = 0, ile = k.i_left_end;
         const double dxT = p.dx * r;
@@ -672,9 +687,17 @@ int rs_tip(rs_ctx* ctx, rs_tip_row* t) {
         const int jT = (int)((j - 1 + dxT) / r);
         const double x = (i - 1.0 + ile) / r - iT;
         const double y = (j - 1.0 + dxT) / r - jT;
-        const double* row0 = Tc.data() + (int64_t)jT * k.NxT_all;
-        const double* row1 = row0 + k.NxT_all;
-        const double f00 = row0[iT], f10 = row0[iT + 1], f01 = row1[iT], f11 = row1[iT + 1];
+        // the four coarse values of the bilinear read
+        double q4[4];
+        const double* Tbuf = ctx->T[ctx->tcur];
+        const int64_t o0 = (int64_t)jT * k.NxT_all + iT, o1 = o0 + k.NxT_all;
+        if (iT < 0 || iT + 1 >= k.NxT_all || jT < 0 || jT + 1 > k.NyT_all)
+            return fail(ctx, RS_ERR_DOMAIN, "tip outside the coarse temperature grid");
+        HIPCHK(ctx, hipMemcpyAsync(q4, Tbuf + o0, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(q4 + 2, Tbuf + o1, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        const double f00 = q4[0], f10 = q4[1], f01 = q4[2], f11 = q4[3];
         const double a10 = f10 - f00, a01 = f01 - f00, a11 = f11 - f10 - f01 + f00;
         Ttip = f00 + a10 * x + a01 * y + a11 * x * y;
         delta = (Ttip - p.Tmelt) / p.deltaT0;

@@ -22,6 +22,9 @@ struct TipFit {
     double x, y, R;  // R in nm, clamped to [1, 1e5]
 };
 int precise_tip(const rs_params& p, const double* psi, TipFit* out);
+// the same fit from per-row data (irow[Ny], vals[8 Ny], as tip_rows_kernel
+// writes them): no field download
+int precise_tip_rows(const rs_params& p, const int* irow, const double* vals, TipFit* out);
 
 // step arguments of a context at its current state (rs_capi.hip)
 StepArgsMixed mixed_step_args(rs_ctx* ctx);
@@ -61,6 +64,8 @@ struct rs_ctx {
     double* x0 = nullptr;
     double* tfine = nullptr;
     double* h_pin = nullptr;  // pinned scalars for D2H
+    double* tipvals = nullptr;  // per-row tip-fit data (8 Ny) and columns (Ny)
+    int* tiprow = nullptr;
     int cur = 0, tcur = 0;
     int band_h = 40, wpb = 4;
     std::string err;

@@ -488,3 +488,45 @@ hipError_t launch_antitrap_f32(const float* psi, const float* psi_n, const doubl
 }
 
 }  // namespace rs
+
+namespace rs {
+
+// Per-row data of the tip fit (one wavefront per row): the last column in
+// [0, Nx-2] with psi > pin (0 if none) by a wavefront max, then psi at that
+// column +-2 (clamped) and at columns 0..2. The host fit needs nothing else
+// (precise_tip_rows), so a tip output moves 36 B per row instead of the field.
+template <typename P>
+__global__ __launch_bounds__(256) void tip_detail_kernel(const P* psi, int Nx, int Ny, double pin,
+                                                         int* irow, double* vals) {
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (j >= Ny) return;
+    const P* row = psi + (int64_t)j * Nx;
+    int best = -1;
+    for (int i = lane; i <= Nx - 2; i += 64)
+        if ((double)row[i] > pin) best = i;
+    for (int o = 32; o >= 1; o >>= 1) best = max(best, __shfl_xor(best, o, 64));
+    if (lane < 8) {
+        const int i0 = best < 0 ? 0 : best;
+        int i = lane < 5 ? i0 - 2 + lane : lane - 5;
+        i = min(max(i, 0), Nx - 1);
+        vals[8 * (int64_t)j + lane] = (double)row[i];
+        if (lane == 0) irow[j] = i0;
+    }
+}
+
+hipError_t launch_tip_detail(const double* psi, int Nx, int Ny, double pin, int* irow,
+                             double* vals, hipStream_t s) {
+    hipLaunchKernelGGL(tip_detail_kernel<double>, dim3((Ny + 3) / 4), dim3(256), 0, s, psi, Nx, Ny,
+                       pin, irow, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_tip_detail_f32(const float* psi, int Nx, int Ny, double pin, int* irow,
+                                 double* vals, hipStream_t s) {
+    hipLaunchKernelGGL(tip_detail_kernel<float>, dim3((Ny + 3) / 4), dim3(256), 0, s, psi, Nx, Ny,
+                       pin, irow, vals);
+    return hipGetLastError();
+}
+
+}  // namespace rs

@@ -115,6 +115,11 @@ hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* 
                        hipStream_t s);
 hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
 hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s);
+// per-row data of the tip fit: irow[Ny], vals[8 Ny] (see precise_tip_rows)
+hipError_t launch_tip_detail(const double* psi, int Nx, int Ny, double pin, int* irow,
+                             double* vals, hipStream_t s);
+hipError_t launch_tip_detail_f32(const float* psi, int Nx, int Ny, double pin, int* irow,
+                                 double* vals, hipStream_t s);
 // antitrapping + solute-trapping correction of c_n (extension; psi in FP64
 // or FP32); vdn = V_D in W/tau0 units (0: no trapping factor)
 hipError_t launch_antitrap(const double* psi, const double* psi_n, const double* c, double* c_n,

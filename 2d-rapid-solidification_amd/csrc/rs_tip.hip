@@ -8,6 +8,8 @@
 // (x0, y0, R).
 #include <math.h>
 
+#include <vector>
+
 #include "rs_host.hpp"
 
 namespace rs {
@@ -131,14 +133,48 @@ int parabola(const double* xs, const double* ys, double* x0, double* y0, double*
 
 }  // namespace
 
-int precise_tip(const rs_params& p, const double* psi, TipFit* out) {
-    const int Nx = p.Nx, Ny = p.Ny;
+// Per-row data of a psi field for the fit: irow[j] = last i in [0, Nx-2]
+// with psi > phi_interface (0 if none); vals[8j..8j+4] = psi at
+// irow-2..irow+2 (clamped to the row), vals[8j+5..8j+7] = psi at 0, 1, 2.
+void tip_rows_host(const rs_params& p, const double* psi, int* irow, double* vals) {
+    const int Nx = p.Nx;
+    for (int j = 0; j < p.Ny; ++j) {
+        const double* row = psi + (int64_t)j * Nx;
+        const int i0 = last_above(p, psi, 0, j);
+        irow[j] = i0;
+        for (int m = 0; m < 5; ++m) {
+            int i = i0 - 2 + m;
+            if (i < 0) i = 0;
+            if (i > Nx - 1) i = Nx - 1;
+            vals[8 * j + m] = row[i];
+        }
+        for (int m = 0; m < 3; ++m) vals[8 * j + 5 + m] = row[m];
+    }
+}
+
+// The fit from the per-row data. A scan started at column s finds the
+// row's last psi > phi_interface at or after s, i.e. irow when irow >= s
+// and column 0 otherwise, so the running scans of fun.cu:2867-2954 are
+// reproduced exactly without the field.
+int precise_tip_rows(const rs_params& p, const int* irow, const double* vals, TipFit* out) {
+    const int Ny = p.Ny;
+    const double pin = p.phi_interface;
+    auto above = [&](int j, int start) {
+        return (irow[j] >= start && vals[8 * j + 2] > pin) ? irow[j] : 0;
+    };
+    // psi at column i0 - 2 + m of row j for the i0 the scan found
+    auto window = [&](int j, int i0, int m) {
+        if (i0 == irow[j]) return vals[8 * j + m];
+        int i = i0 - 2 + m;  // i0 == 0
+        if (i < 0) i = 0;
+        return vals[8 * j + 5 + i];
+    };
     double itip = 0.0;
     int bi = 0, bj = 0;
     for (int j = 0; j < Ny; ++j) {
-        const int i0 = last_above(p, psi, (int)itip, j);
-        const double a = psi[(int64_t)j * Nx + i0], b = psi[(int64_t)j * Nx + i0 + 1];
-        const double est = (b - a) * (p.phi_interface - a) + i0;
+        const int i0 = above(j, (int)itip);
+        const double a = window(j, i0, 2), b = window(j, i0, 3);
+        const double est = (b - a) * (pin - a) + i0;
         if (est > itip) {
             itip = est;
             bi = (int)itip;
@@ -149,16 +185,13 @@ int precise_tip(const rs_params& p, const double* psi, TipFit* out) {
     for (int q = 0; q < 5; ++q) {
         const int j0 = wrap_row(bj - 2 + q, Ny - 2, 1);
         const int start = bi - 5 < 0 ? 0 : bi - 5;
-        const int i0 = last_above(p, psi, start, j0);
+        const int i0 = above(j0, start);
         double id[5], pd[5];
         for (int m = 0; m < 5; ++m) {
-            int i = i0 - 2 + m;
-            id[m] = i;
-            if (i < 0) i = 0;
-            if (i > Nx - 1) i = Nx - 1;
-            pd[m] = psi[(int64_t)j0 * Nx + i];
+            id[m] = i0 - 2 + m;
+            pd[m] = window(j0, i0, m);
         }
-        xd[q] = quartic_at(id, pd, p.phi_interface);
+        xd[q] = quartic_at(id, pd, pin);
         yd[q] = j0;
     }
     double x0 = NAN, y0 = NAN, R = NAN;
@@ -172,6 +205,13 @@ int precise_tip(const rs_params& p, const double* psi, TipFit* out) {
     if (Rn < 1.0) Rn = 1.0;
     out->R = Rn;
     return st;
+}
+
+int precise_tip(const rs_params& p, const double* psi, TipFit* out) {
+    std::vector<int> irow((size_t)p.Ny);
+    std::vector<double> vals((size_t)p.Ny * 8);
+    tip_rows_host(p, psi, irow.data(), vals.data());
+    return precise_tip_rows(p, irow.data(), vals.data(), out);
 }
 
 }  // namespace rs

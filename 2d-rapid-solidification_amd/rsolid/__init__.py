@@ -77,6 +77,19 @@ def _load() -> C.CDLL:
         "rs_batch_surrogate_train": (C.c_int, [vp, P(RsSurrogateOpts), P(C.c_double), C.c_int32]),
         "rs_batch_tune": (C.c_int, [vp, C.c_int32, P(C.c_double), C.c_int32]),
         "rs_batch_info": (C.c_int, [vp, P(C.c_double), C.c_int32]),
+        "rs_write_field_txt": (C.c_int, [C.c_char_p, P(C.c_double), C.c_int32, C.c_int32, C.c_int32,
+                                         C.c_int32]),
+        "rs_write_tip_row": (C.c_int, [C.c_char_p, P(RsTipRow), C.c_int32]),
+        "rs_append_offset": (C.c_int, [C.c_char_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32]),
+        "rs_append_history": (C.c_int, [C.c_char_p, P(C.c_double), C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_int32, C.c_int32, P(C.c_int32)]),
+        "rs_write_checkpoint": (C.c_int, [C.c_char_p, P(C.c_double), C.c_int64]),
+        "rs_read_checkpoint": (C.c_int, [C.c_char_p, P(C.c_double), C.c_int64]),
+        "rs_write_parameters": (C.c_int, [C.c_char_p, P(RsParams), P(RsConstants), C.c_int32]),
+        "rs_read_parameters": (C.c_int, [C.c_char_p, P(RsParams), P(RsConstants)]),
+        "rs_download_columns": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
+        "rs_save_state": (C.c_int, [vp, C.c_char_p]),
+        "rs_load_state": (C.c_int, [vp, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -118,6 +131,27 @@ def ensemble_params(N: int, deltaT: float, Vp: float, precision: int = RS_PREC_M
     p.deltaT0 = deltaT
     p.Vp = Vp
     return p
+
+
+def write_field_txt(path: str, data: np.ndarray, di: int = 1, dj: int = 1) -> None:
+    a = np.ascontiguousarray(data, dtype=np.float64)
+    rc = lib.rs_write_field_txt(path.encode(), _dptr(a), a.shape[1], a.shape[0], di, dj)
+    if rc:
+        raise RsError(rc, f"rs_write_field_txt {path}")
+
+
+def write_parameters(path: str, p: RsParams, k: RsConstants, i_tip: int = 0) -> None:
+    rc = lib.rs_write_parameters(path.encode(), C.byref(p), C.byref(k), i_tip)
+    if rc:
+        raise RsError(rc, f"rs_write_parameters {path}")
+
+
+def read_parameters(path: str, p: RsParams) -> RsConstants:
+    k = derive(p)
+    rc = lib.rs_read_parameters(path.encode(), C.byref(p), C.byref(k))
+    if rc:
+        raise RsError(rc, f"rs_read_parameters {path}")
+    return k
 
 
 def derive(p: RsParams) -> RsConstants:
@@ -309,6 +343,19 @@ class Context:
         v = (C.c_double * 3)()
         self._chk(lib.rs_surrogate_stats(self._h, v, 1 if reset else 0), "rs_surrogate_stats")
         return {"fallback": v[0], "evaluated": v[1], "fallback_frac": v[2]}
+
+    # -- formats -------------------------------------------------------------
+    def download_columns(self, field: int, i0: int, ncols: int) -> np.ndarray:
+        out = np.empty((self.p.Ny, ncols), dtype=np.float64)
+        self._chk(lib.rs_download_columns(self._h, field, i0, ncols, _dptr(out)),
+                  "rs_download_columns")
+        return out
+
+    def save_state(self, path: str) -> None:
+        self._chk(lib.rs_save_state(self._h, path.encode()), "rs_save_state")
+
+    def load_state(self, path: str) -> None:
+        self._chk(lib.rs_load_state(self._h, path.encode()), "rs_load_state")
 
     def timing(self, which: int) -> tuple[float, int]:
         ms = C.c_double()

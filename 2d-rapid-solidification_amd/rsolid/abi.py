@@ -106,4 +106,7 @@ EXPORTED = [
     "rs_surrogate_get", "rs_surrogate_set", "rs_surrogate_grad", "rs_surrogate_eval",
     "rs_surrogate_stats", "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error",
     "rs_batch_run", "rs_batch_step", "rs_batch_surrogate_train", "rs_batch_tune", "rs_batch_info",
+    "rs_write_field_txt", "rs_write_tip_row", "rs_append_offset", "rs_append_history",
+    "rs_write_checkpoint", "rs_read_checkpoint", "rs_write_parameters", "rs_read_parameters",
+    "rs_download_columns", "rs_save_state", "rs_load_state",
 ]

@@ -253,6 +253,44 @@ int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
  * field contents; writes the average kernel time (HIP events). */
 int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms);
 
+/* ---- on-disk formats of the reference's driver (host only) --------------
+ * The files main() writes, so the reference's post-processing and restart
+ * read this build's output unchanged. */
+/* save_field (fun.cu:1768-1790): "%g\t" per value, one line per row, every
+ * di-th column and dj-th row */
+int rs_write_field_txt(const char* path, const double* data, int32_t Nx, int32_t Ny, int32_t di,
+                       int32_t dj);
+/* one tip.txt row (src/main.cu:2671-2709), appended; the two header lines
+ * first when header != 0 */
+int rs_write_tip_row(const char* path, const rs_tip_row* t, int32_t header);
+/* offset.txt record of a pull-back (fun.cu:710-716), appended */
+int rs_append_offset(const char* path, int64_t step, int32_t i_offset, int32_t i_offset_history,
+                     int32_t step_pull_back);
+/* c_history_temp.txt records of the columns a pull-back shifts out
+ * (fun.cu:719-740); cols is Ny x ncols row-major (rs_download_columns);
+ * *i_ignore is the reference's skip counter, updated */
+int rs_append_history(const char* path, const double* cols, int32_t ncols, int32_t Ny,
+                      int32_t i_offset_history, int32_t di, int32_t dj, int32_t* i_ignore);
+/* raw FP64 checkpoints (save_check_point / load_check_point,
+ * fun.cu:1848-1916): n doubles in memory order */
+int rs_write_checkpoint(const char* path, const double* data, int64_t n);
+int rs_read_checkpoint(const char* path, double* data, int64_t n);
+/* parameters.txt (save_parameters fun.cu:2060-2202, same lines and order);
+ * rs_read_parameters fills the restart fields load_parameters
+ * (fun.cu:2204-2388) takes from it into k (derived constants on entry) */
+int rs_write_parameters(const char* path, const rs_params* p, const rs_constants* k, int32_t i_tip);
+int rs_read_parameters(const char* path, const rs_params* p, rs_constants* k);
+/* columns [i0, i0+ncols) of RS_FIELD_C or RS_FIELD_PSI into host
+ * (Ny x ncols, row-major): the columns a pull-back shifts out, without a
+ * whole-field copy */
+int rs_download_columns(rs_ctx* ctx, int32_t field, int32_t i0, int32_t ncols, double* host);
+/* Exact state file (no counterpart in the reference, whose restart goes
+ * through %g text): run state, parities and the device fields in their
+ * stored precision; loading into a context created from the same
+ * parameters continues the run bit for bit. */
+int rs_save_state(rs_ctx* ctx, const char* path);
+int rs_load_state(rs_ctx* ctx, const char* path);
+
 /* ---- learned CALPHAD surrogate ------------------------------------------
  * No counterpart in the reference, which evaluates Gl/Gs/Glc/Gsc
  * (fun.cu:1671-1720) exactly at every use. A 2 -> 64 -> 64 -> 4 MLP maps
