@@ -140,7 +140,7 @@ int build_tables(rs_batch* b) {
     L.ident = ident;
     const rs_ctx* c0 = b->runs[0];
     L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
-    if (L.surr) L.surr = b->h_runs[0].a.s_split ? 2 : 1;
+    if (L.surr) L.surr = 1 + b->h_runs[0].a.s_fmt;
     const int res = batch_step_resident(L);
     L.step_blocks = std::max(1, std::min(L.items, res * b->cus * b->grid_mult));
     L.T_blocks = std::max(1, std::min((L.cells + 255) / 256, 8 * b->cus));

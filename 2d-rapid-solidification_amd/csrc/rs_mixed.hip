@@ -204,7 +204,7 @@ struct SurrLds {
     const uint64_t* trust;
 };
 
-template <bool SPLIT>
+template <int FMT>
 __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* lntab,
                                          const SurrLds& sl, float Tp, double c, float h, int i,
                                          int j, bool real, float& dG, double& mu, unsigned& nfb,
@@ -219,7 +219,7 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
     asm volatile("" : "+v"(zoff));
     const MlpInfer& F =
         *reinterpret_cast<const MlpInfer*>(reinterpret_cast<const char*>(sl.F) + zoff);
-    const mlp::f4 y = mlp::forward<SPLIT>(F, mlp::input_frag(ain, bin), lane);
+    const mlp::f4 y = mlp::forward<FMT>(F, mlp::input_frag(ain, bin), lane);
     bool ok = fabsf(ain) <= 1.0f && fabsf(bin) <= 1.0f && i > 1 && i < a.Nx - 2 && j > 1 &&
               j < a.Ny - 2;
     if (ok) {
@@ -262,7 +262,7 @@ __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const doubl
     const float Tp = temp_m(a, rt, i);
     const float h = 0.5f * (1.0f + gg<float>(psi));
     if constexpr (SURR != 0)
-        thermo_s<SURR == 2>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb, nall);
+        thermo_s<SURR - 1>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb, nall);
     else
         thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
     return q;
@@ -360,6 +360,8 @@ struct StepLds<1> {
 };
 template <>
 struct StepLds<2> : StepLds<1> {};
+template <>
+struct StepLds<3> : StepLds<1> {};
 
 template <int SURR>
 __device__ __forceinline__ void load_step_lds(StepLds<SURR>& L, const MlpInfer* mlp,
@@ -492,8 +494,8 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
     }
 }
 
-// SURR: 0 exact thermodynamics, 1 surrogate, 2 surrogate with split
-// activations. At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
+// SURR: 0 exact thermodynamics, else the surrogate with activation format
+// SURR - 1 (kActBf16, kActSplit, kActFp16). At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
 // the exact thermodynamics: the kernel is latency-bound below that. The
 // surrogate variants hold the MFMA operands and run at 2.
 template <bool IDENT, int SURR>
@@ -518,7 +520,12 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    if (surr && a.s_split) {
+    if (surr && a.s_fmt == kActFp16) {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 3>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 3>), grid, block, 0, s, a);
+    } else if (surr && a.s_fmt == kActSplit) {
         if (ident)
             hipLaunchKernelGGL((step_mixed_kernel<true, 2>), grid, block, 0, s, a);
         else
@@ -641,7 +648,9 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
     case 2: e = batch_step_t<false, 1>(L, s); break;
     case 3: e = batch_step_t<true, 1>(L, s); break;
     case 4: e = batch_step_t<false, 2>(L, s); break;
-    default: e = batch_step_t<true, 2>(L, s); break;
+    case 5: e = batch_step_t<true, 2>(L, s); break;
+    case 6: e = batch_step_t<false, 3>(L, s); break;
+    default: e = batch_step_t<true, 3>(L, s); break;
     }
     if (e != hipSuccess) return e;
     if (L.cells > 0) {
@@ -669,7 +678,9 @@ int batch_step_resident(const BatchLaunch& L) {
     case 2: return resident_t<false, 1>(L.wpb);
     case 3: return resident_t<true, 1>(L.wpb);
     case 4: return resident_t<false, 2>(L.wpb);
-    default: return resident_t<true, 2>(L.wpb);
+    case 5: return resident_t<true, 2>(L.wpb);
+    case 6: return resident_t<false, 3>(L.wpb);
+    default: return resident_t<true, 3>(L.wpb);
     }
 }
 

@@ -32,14 +32,15 @@ struct StepArgsMixed {
     double L0l, L1l, RR;
     float dL0, dL1;          // L0_liq - L0_bcc, L1_liq - L1_bcc
     RK rk;                   // exact routine for cells outside the table
-    // learned surrogate (surrogate kernel only): BF16 fragments, trust
-    // bits, fallback counters [fallback, all], input/output normalisation
+    // learned surrogate (surrogate kernel only): weight fragments of the
+    // activation format, trust bits, fallback counters [fallback, all],
+    // input/output normalisation
     const MlpInfer* mlp;
     const uint64_t* trust;
     unsigned long long* fb_count;
     float s_a_scale, s_a_off;  // a = T' s_a_scale + s_a_off
     float y_mean[4], y_std[4];
-    int s_split;               // split (hi + lo) BF16 activations
+    int s_fmt;                 // activation format (kActBf16, kActSplit, kActFp16)
 };
 
 // ident: the crystal orientation is the identity (the reference's only
@@ -90,7 +91,7 @@ struct BatchLaunch {
     int nruns, items, cells;
     int step_blocks, wpb, T_blocks;
     bool ident;
-    int surr;           // 0 exact, 1 surrogate, 2 split-activation surrogate
+    int surr;           // 0 exact, else 1 + the surrogate's activation format
 };
 // one step of every run: fused step kernel, coarse T kernel, state advance
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);

@@ -1,4 +1,4 @@
-// rs_mlp.hip -- BF16-MFMA training and inference kernels of the CALPHAD
+// rs_mlp.hip -- MFMA training (BF16) and inference (BF16, FP16) kernels of the CALPHAD
 // surrogate (layouts and conventions in rs_mlp.hpp), and its exact
 // training-sample generator.
 #include <math.h>
@@ -21,6 +21,13 @@ __device__ __forceinline__ uint2 pack_k16(const float* v) {
 }
 __device__ __forceinline__ uint4 pack_k32(const float* v) {
     return make_uint4(cvt2(v[0], v[1]), cvt2(v[2], v[3]), cvt2(v[4], v[5]), cvt2(v[6], v[7]));
+}
+// the BF16 values of v held in FP16 (exact for |v| >= 2^-14; below, FP16
+// subnormals round them)
+__device__ __forceinline__ uint4 pack_k32_h(const float* v) {
+    float r[8];
+    for (int e = 0; e < 8; ++e) r[e] = bf_lo(cvt2(v[e], 0.f));
+    return make_uint4(cvt2h(r[0], r[1]), cvt2h(r[2], r[3]), cvt2h(r[4], r[5]), cvt2h(r[6], r[7]));
 }
 
 template <typename T>
@@ -64,6 +71,7 @@ __global__ void mlp_pack_kernel(const float* P, MlpFrags* F) {
             const int m = 16 * mo + m16;
             for (int e = 0; e < 8; ++e) v[e] = W2[m * kH + sigma(kb, g, e)];
             F->f.w2[mo][kb][lane] = pack_k32(v);
+            F->h.w2[mo][kb][lane] = pack_k32_h(v);
             for (int e = 0; e < 8; ++e) v[e] = W2[sigma(kb, g, e) * kH + m];
             F->w2t[mo][kb][lane] = pack_k32(v);
         }
@@ -73,10 +81,12 @@ __global__ void mlp_pack_kernel(const float* P, MlpFrags* F) {
             const int o = m16 - 4 * t;
             for (int e = 0; e < 8; ++e) v[e] = on ? W3[o * kH + sigma(kb, g, e)] : 0.f;
             F->f.w3[t][kb][lane] = pack_k32(v);
+            F->h.w3[t][kb][lane] = pack_k32_h(v);
         }
-    F->f.b1[lane] = P[kOffB1 + lane];
-    F->f.b2[lane] = P[kOffB2 + lane];
-    if (lane < 4) F->f.b3[lane] = P[kOffB3 + lane];
+    for (int mt = 0; mt < 4; ++mt) F->h.w1[mt][lane] = F->f.w1[mt][lane];
+    F->f.b1[lane] = F->h.b1[lane] = P[kOffB1 + lane];
+    F->f.b2[lane] = F->h.b2[lane] = P[kOffB2 + lane];
+    if (lane < 4) F->f.b3[lane] = F->h.b3[lane] = P[kOffB3 + lane];
 }
 
 hipError_t launch_mlp_pack(const float* P, MlpFrags* F, hipStream_t s) {
@@ -95,12 +105,12 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(const float* X, const flo
                                                        int64_t row0, int64_t mult, int64_t off,
                                                        float gscale, const MlpFrags* Fg,
                                                        float* partial, float* loss_part) {
-    __shared__ MlpFrags F;
+    __shared__ MlpTrain F;
     // transposed BF16 tiles [feature][sample of the tile]
     __shared__ __align__(16) uint16_t sH1[kH][16], sH2[kH][16], sZ1[kH][16], sZ2[kH][16];
     __shared__ __align__(16) uint16_t sX[16][16], sY[16][16];
     const int lane = threadIdx.x;
-    copy_words(&F, Fg, lane, 64);
+    copy_words<MlpTrain>(&F, Fg, lane, 64);
     for (int q = lane; q < 16 * 16; q += 64) {
         (&sX[0][0])[q] = 0;
         (&sY[0][0])[q] = 0;
@@ -132,7 +142,7 @@ __global__ __launch_bounds__(64) void mlp_train_kernel(const float* X, const flo
     for (int t = 0; t < 4; ++t) {
         // forward of tile t (its outputs land in lanes 16t..16t+15)
         f4 z1[4], z2[4];
-        Acts<SPLIT> h1, h2;
+        Acts<SPLIT ? kActSplit : kActBf16> h1, h2;
         layer1(F.f, xin, t, lane, z1);
         activate(z1, h1);
         layer2(F.f, h1, lane, z2);
@@ -351,28 +361,32 @@ hipError_t launch_mlp_grad(const float* X, const float* Yn, int n, const MlpFrag
 // ---------------------------------------------------------------------------
 // inference over rows (tests, trust table): the fused stepper's forward
 // ---------------------------------------------------------------------------
-template <bool SPLIT>
+template <int FMT>
 __global__ __launch_bounds__(256) void mlp_eval_kernel(const float* X, int64_t n,
                                                        const MlpFrags* Fg, float* out) {
     __shared__ MlpInfer F;
-    copy_words(&F, &Fg->f, threadIdx.x, blockDim.x);
+    copy_words(&F, FMT == kActFp16 ? &Fg->h : &Fg->f, threadIdx.x, blockDim.x);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64 + lane;
     const int64_t rc = r < n ? r : n - 1;
-    const f4 y = forward<SPLIT>(F, input_frag(X[2 * rc], X[2 * rc + 1]), lane);
+    const f4 y = forward<FMT>(F, input_frag(X[2 * rc], X[2 * rc + 1]), lane);
     if (r < n)
         for (int o = 0; o < 4; ++o) out[4 * r + o] = y[o];
 }
 
-hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, bool split,
+hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, int fmt,
                            hipStream_t s) {
     const int64_t waves = (n + 63) / 64;
     const dim3 grid((unsigned)((waves + 3) / 4));
-    if (split)
-        hipLaunchKernelGGL(mlp_eval_kernel<true>, grid, dim3(256), 0, s, X, n, F, out);
+    if (fmt == kActFp16)
+        hipLaunchKernelGGL(mlp_eval_kernel<kActFp16>, grid, dim3(256), 0, s, X, n, F, out);
+    else if (fmt == kActSplit)
+        hipLaunchKernelGGL(mlp_eval_kernel<kActSplit>, grid, dim3(256), 0, s, X, n, F, out);
+    else if (fmt == kActBf16)
+        hipLaunchKernelGGL(mlp_eval_kernel<kActBf16>, grid, dim3(256), 0, s, X, n, F, out);
     else
-        hipLaunchKernelGGL(mlp_eval_kernel<false>, grid, dim3(256), 0, s, X, n, F, out);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

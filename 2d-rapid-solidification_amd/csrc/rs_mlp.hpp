@@ -14,10 +14,15 @@
 // RT x ln x terms are added back exactly in FP64 by the user.
 // Activation: SiLU in its hard form, act(x) = x * clamp(x/6 + 1/2, 0, 1)
 // (two VALU instructions: FMA with the output clamp, multiply).
-// Split activations (SPLIT = true): each hidden activation enters the next
-// product as two BF16 parts, h = hi + lo, lo = bf16(h - hi), i.e. with ~16
-// significant bits instead of 8; the product costs two MFMAs. Weights stay
-// BF16 either way.
+// Activation formats (FMT) of the hidden layers, computed in FP32 and
+// rounded once where they enter the next product:
+//   kActBf16  BF16 (8 significant bits), BF16 MFMA;
+//   kActSplit two BF16 parts, h = hi + lo, lo = bf16(h - hi) (~16 bits),
+//             two BF16 MFMAs per product;
+//   kActFp16  FP16 (11 bits; one v_cvt_pk_f16_f32 per pair), FP16 MFMA for
+//             layers 2 and 3 (weights: their BF16 values held in FP16).
+// Weights are BF16 values in every format, layer 1 is BF16 MFMA in every
+// format, and training runs the BF16 formats.
 //
 // Wavefront layout ("home lanes"): each lane owns one sample (a stencil
 // cell). The 64 samples of a wavefront form 4 tiles of 16: tile t is lanes
@@ -70,7 +75,10 @@ constexpr int kNQ = kQB3 + 16;  // 6288
 // trust table: 64 x 64 bins over (a, b) in [-1, 1]^2, bit ia of word ib
 constexpr int kTrustN = 64;
 
-// BF16 weight fragments for inference (20.8 KB, staged in LDS)
+enum { kActBf16 = 0, kActSplit = 1, kActFp16 = 2 };
+
+// weight fragments for inference (20.8 KB, staged in LDS); w1 is BF16, w2
+// and w3 BF16 (f) or FP16 (h) bits
 struct MlpInfer {
     uint4 w1[4][64];     // [m-tile]            K32, k slots of lane group 0
     uint4 w2[4][2][64];  // [m-tile][k-block]   K32
@@ -78,10 +86,14 @@ struct MlpInfer {
     float b1[kH], b2[kH], b3[4];
 };
 // training adds the transposed fragments of the backward pass
-struct MlpFrags {
+struct MlpTrain {
     MlpInfer f;
     uint2 w3t[4][4][64];  // [tile][m-tile]     K16: W3^T, k-group = tile
     uint4 w2t[4][2][64];  // [m-tile][k-block]  K32: W2^T
+};
+// all fragments; h is the inference set of the FP16 format
+struct MlpFrags : MlpTrain {
+    MlpInfer h;
 };
 
 struct MlpNorm {
@@ -130,7 +142,7 @@ hipError_t launch_mlp_grad(const float* X, const float* Yn, int n, const MlpFrag
                            float* partial, float* loss_part, float* grad, bool split,
                            hipStream_t s);
 // forward over n rows -> normalised outputs (n x 4)
-hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, bool split,
+hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, int fmt,
                            hipStream_t s);
 
 // ---------------------------------------------------------------------------
@@ -142,6 +154,8 @@ namespace mlp {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -153,6 +167,18 @@ __device__ __forceinline__ uint32_t cvt2(float a, float b) {
 __device__ __forceinline__ uint2 pack4(f4 v) {
     return make_uint2(cvt2(v[0], v[1]), cvt2(v[2], v[3]));
 }
+// two floats -> packed FP16 pair, round to nearest even (v_cvt_pk_f16_f32)
+__device__ __forceinline__ uint32_t cvt2h(float a, float b) {
+    const f2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, h2));
+}
+template <int FMT>
+__device__ __forceinline__ uint2 pack4f(f4 v) {
+    if constexpr (FMT == kActFp16)
+        return make_uint2(cvt2h(v[0], v[1]), cvt2h(v[2], v[3]));
+    else
+        return pack4(v);
+}
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -163,6 +189,18 @@ __device__ __forceinline__ f4 mfma16(uint2 a, uint2 b, f4 c) {
 __device__ __forceinline__ f4 mfma32(uint4 a, uint4 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
                                                    __builtin_bit_cast(b8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma32h(uint4 a, uint4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, a),
+                                                  __builtin_bit_cast(h8, b), c, 0, 0, 0);
+}
+// the product of a hidden layer in format FMT
+template <int FMT>
+__device__ __forceinline__ f4 mfma32f(uint4 a, uint4 b, f4 c) {
+    if constexpr (FMT == kActFp16)
+        return mfma32h(a, b, c);
+    else
+        return mfma32(a, b, c);
 }
 __device__ __forceinline__ uint4 cat(uint2 lo, uint2 hi) { return make_uint4(lo.x, lo.y, hi.x, hi.y); }
 
@@ -216,32 +254,33 @@ __device__ __forceinline__ f4 unpack4(uint2 w) {
     return f4{bf_lo(w.x), bf_hi(w.x), bf_lo(w.y), bf_hi(w.y)};
 }
 
-// BF16 activations of one layer as 4 D tiles (hi part; lo part if split)
-template <bool SPLIT>
+// activations of one layer as 4 D tiles (hi: the BF16 or FP16 value; lo:
+// the BF16 remainder of the split format)
+template <int FMT>
 struct Acts {
     uint2 hi[4];
-    uint2 lo[SPLIT ? 4 : 1];
+    uint2 lo[FMT == kActSplit ? 4 : 1];
 };
-template <bool SPLIT>
-__device__ __forceinline__ void activate(const f4* z, Acts<SPLIT>& h) {
+template <int FMT>
+__device__ __forceinline__ void activate(const f4* z, Acts<FMT>& h) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
         const f4 a = act4(z[mt]);
-        h.hi[mt] = pack4(a);
-        if constexpr (SPLIT) h.lo[mt] = pack4(a - unpack4(h.hi[mt]));
+        h.hi[mt] = pack4f<FMT>(a);
+        if constexpr (FMT == kActSplit) h.lo[mt] = pack4(a - unpack4(h.hi[mt]));
     }
 }
 
 // layer 2: z2[mo] = W2 h1 + b2
-template <bool SPLIT>
-__device__ __forceinline__ void layer2(const MlpInfer& F, const Acts<SPLIT>& h1, int lane, f4* z2) {
+template <int FMT>
+__device__ __forceinline__ void layer2(const MlpInfer& F, const Acts<FMT>& h1, int lane, f4* z2) {
     const uint4 k0 = cat(h1.hi[0], h1.hi[1]), k1 = cat(h1.hi[2], h1.hi[3]);
 #pragma unroll
     for (int mo = 0; mo < 4; ++mo) {
         f4 acc = bias4(F.b2, mo, lane);
-        acc = mfma32(F.w2[mo][0][lane], k0, acc);
-        acc = mfma32(F.w2[mo][1][lane], k1, acc);
-        if constexpr (SPLIT) {
+        acc = mfma32f<FMT>(F.w2[mo][0][lane], k0, acc);
+        acc = mfma32f<FMT>(F.w2[mo][1][lane], k1, acc);
+        if constexpr (FMT == kActSplit) {
             acc = mfma32(F.w2[mo][0][lane], cat(h1.lo[0], h1.lo[1]), acc);
             acc = mfma32(F.w2[mo][1][lane], cat(h1.lo[2], h1.lo[3]), acc);
         }
@@ -249,12 +288,12 @@ __device__ __forceinline__ void layer2(const MlpInfer& F, const Acts<SPLIT>& h1,
     }
 }
 // layer 3 of tile t accumulated into the home lanes of tile t
-template <bool SPLIT>
-__device__ __forceinline__ f4 layer3(const MlpInfer& F, const Acts<SPLIT>& h2, int t, int lane,
+template <int FMT>
+__device__ __forceinline__ f4 layer3(const MlpInfer& F, const Acts<FMT>& h2, int t, int lane,
                                      f4 acc) {
-    acc = mfma32(F.w3[t][0][lane], cat(h2.hi[0], h2.hi[1]), acc);
-    acc = mfma32(F.w3[t][1][lane], cat(h2.hi[2], h2.hi[3]), acc);
-    if constexpr (SPLIT) {
+    acc = mfma32f<FMT>(F.w3[t][0][lane], cat(h2.hi[0], h2.hi[1]), acc);
+    acc = mfma32f<FMT>(F.w3[t][1][lane], cat(h2.hi[2], h2.hi[3]), acc);
+    if constexpr (FMT == kActSplit) {
         acc = mfma32(F.w3[t][0][lane], cat(h2.lo[0], h2.lo[1]), acc);
         acc = mfma32(F.w3[t][1][lane], cat(h2.lo[2], h2.lo[3]), acc);
     }
@@ -262,14 +301,15 @@ __device__ __forceinline__ f4 layer3(const MlpInfer& F, const Acts<SPLIT>& h2, i
 }
 
 // full forward of the wavefront's 64 samples; returns the lane's own
-// normalised outputs y0..y3
-template <bool SPLIT>
+// normalised outputs y0..y3 (F: the set of format FMT, MlpFrags::h for
+// kActFp16)
+template <int FMT>
 __device__ __forceinline__ f4 forward(const MlpInfer& F, uint4 xin, int lane) {
     f4 y = {F.b3[0], F.b3[1], F.b3[2], F.b3[3]};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         f4 z[4];
-        Acts<SPLIT> h;
+        Acts<FMT> h;
         layer1(F, xin, t, lane, z);
         activate(z, h);
         layer2(F, h, lane, z);

@@ -24,7 +24,7 @@ struct SurrState {
     unsigned long long* count = nullptr;     // fallback, all evaluations
     MlpNorm norm{};
     double T_lo = 0.0, T_hi = 0.0, tol = 1e-4;
-    bool split = true;  // split (hi + lo) BF16 activations
+    int fmt = kActFp16;  // inference activation format (training: BF16, split unless kActBf16)
     uint64_t h_trust[kTrustN] = {};
     bool ready = false;
 };
@@ -140,7 +140,7 @@ int build_trust(rs_ctx* ctx, double* max_err4, double* max_res_trusted, double* 
     HIPCHK(ctx, sc.get(&dO, 4 * n));
     HIPCHK(ctx, hipMemcpyAsync(dTc, tc.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, launch_mlp_inputs(dTc, n, s->norm.t_mid, s->norm.t_half, dX, ctx->stream));
-    HIPCHK(ctx, launch_mlp_eval(dX, n, s->F, dO, s->split, ctx->stream));
+    HIPCHK(ctx, launch_mlp_eval(dX, n, s->F, dO, s->fmt, ctx->stream));
     HIPCHK(ctx, launch_mlp_targets(dTc, n, ctx->p.L0_bcc, ctx->p.L0_liq, ctx->p.L1_bcc, ctx->p.L1_liq,
                                    dY, ctx->stream));
     std::vector<double> Y((size_t)(4 * n));
@@ -183,7 +183,7 @@ int build_trust(rs_ctx* ctx, double* max_err4, double* max_res_trusted, double* 
 
 // mean squared error (normalised outputs) of the network over a sample set
 int mse_of(rs_ctx* ctx, const float* dX, const float* dYn, int64_t n, float* dO, double* mse) {
-    HIPCHK(ctx, launch_mlp_eval(dX, n, ctx->surr->F, dO, ctx->surr->split, ctx->stream));
+    HIPCHK(ctx, launch_mlp_eval(dX, n, ctx->surr->F, dO, ctx->surr->fmt, ctx->stream));
     std::vector<float> O((size_t)(4 * n)), Yn((size_t)(4 * n));
     HIPCHK(ctx, hipMemcpyAsync(O.data(), dO, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(Yn.data(), dYn, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -215,7 +215,7 @@ bool surr_ready(const rs_ctx* ctx) { return ctx->surr && ctx->surr->ready; }
 
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
     const SurrState* s = ctx->surr;
-    a.mlp = &s->F->f;
+    a.mlp = s->fmt == kActFp16 ? &s->F->h : &s->F->f;
     a.trust = s->trust;
     a.fb_count = s->count;
     // a = (T - t_mid)/t_half with T = Tref + T'
@@ -225,7 +225,7 @@ void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
         a.y_mean[k] = s->norm.y_mean[k];
         a.y_std[k] = s->norm.y_std[k];
     }
-    a.s_split = s->split ? 1 : 0;
+    a.s_fmt = s->fmt;
 }
 
 }  // namespace rs
@@ -246,7 +246,7 @@ int rs_surrogate_defaults(rs_surrogate_opts* o) {
     o->lr_final = 3e-5;
     o->residual_tol = 1e-4;
     o->n_curve = 256;
-    o->split_act = 1;
+    o->act_format = RS_ACT_FP16;
     return RS_OK;
 }
 
@@ -260,6 +260,8 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
         return fail(ctx, RS_ERR_ARG, "batch must be a multiple of 64 and at most the sample count");
     if (o->epochs < 0 || o->n_curve < 2 || !(o->residual_tol > 0.0) || !(o->lr > 0.0))
         return fail(ctx, RS_ERR_ARG, "bad surrogate options");
+    if (o->act_format < RS_ACT_BF16 || o->act_format > RS_ACT_FP16)
+        return fail(ctx, RS_ERR_ARG, "act_format must be RS_ACT_BF16, RS_ACT_BF16_SPLIT or RS_ACT_FP16");
     int rc = ensure_state(ctx);
     if (rc) return rc;
     SurrState* s = ctx->surr;
@@ -290,7 +292,7 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
     s->T_lo = lo;
     s->T_hi = hi;
     s->tol = o->residual_tol;
-    s->split = o->split_act != 0;
+    s->fmt = o->act_format;
     s->norm.t_mid = (float)(0.5 * (lo + hi));
     s->norm.t_half = (float)(0.5 * (hi - lo));
 
@@ -429,7 +431,7 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
             const double lr = o->lr_final + 0.5 * (o->lr - o->lr_final) * (1.0 + cos(M_PI * fr));
             hipError_t e = launch_mlp_train_step(dX, dYn, n, b * o->batch, o->batch, mult, off, s->F,
                                                  partial, lossp, s->P, s->m, s->v, (float)lr, 0.9f,
-                                                 0.999f, 1e-8f, (int)t, s->split, ctx->stream);
+                                                 0.999f, 1e-8f, (int)t, s->fmt != kActBf16, ctx->stream);
             if (e != hipSuccess) {
                 (void)hipEventDestroy(e0);
                 (void)hipEventDestroy(e1);
@@ -472,7 +474,7 @@ int rs_surrogate_get(rs_ctx* ctx, float* params, int32_t np, double* meta, int32
                                          s->norm.y_mean[1], s->norm.y_mean[2], s->norm.y_mean[3],
                                          s->norm.y_std[0],  s->norm.y_std[1],  s->norm.y_std[2],
                                          s->norm.y_std[3],  s->T_lo,          s->T_hi,
-                                         s->tol,            s->split ? 1.0 : 0.0};
+                                         s->tol,            (double)s->fmt};
         for (int q = 0; q < nm && q < RS_SURR_NMETA; ++q) meta[q] = v[q];
     }
     return RS_OK;
@@ -482,6 +484,8 @@ int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double*
     if (!ctx || !params || !meta) return RS_ERR_ARG;
     if (np != kNParam || nm < RS_SURR_NMETA) return fail(ctx, RS_ERR_ARG, "parameter or meta count");
     if (!(meta[1] > 0.0) || !(meta[12] > 0.0)) return fail(ctx, RS_ERR_ARG, "bad surrogate meta");
+    if (!(meta[13] == RS_ACT_BF16 || meta[13] == RS_ACT_BF16_SPLIT || meta[13] == RS_ACT_FP16))
+        return fail(ctx, RS_ERR_ARG, "bad surrogate meta: act_format");
     for (int k = 0; k < 4; ++k)
         if (!(meta[6 + k] > 0.0)) return fail(ctx, RS_ERR_ARG, "bad surrogate meta");
     int rc = ensure_state(ctx);
@@ -497,7 +501,7 @@ int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double*
     s->T_lo = meta[10];
     s->T_hi = meta[11];
     s->tol = meta[12];
-    s->split = meta[13] != 0.0;
+    s->fmt = (int)meta[13];
     HIPCHK(ctx, hipMemcpyAsync(s->P, params, sizeof(float) * kNParam, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(s->m, 0, sizeof(float) * kNParam, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(s->v, 0, sizeof(float) * kNParam, ctx->stream));
@@ -524,7 +528,7 @@ int rs_surrogate_grad(rs_ctx* ctx, const float* X, const float* Yn, int32_t n, f
     HIPCHK(ctx, sc.get(&dg, kNParam));
     HIPCHK(ctx, hipMemcpyAsync(dX, X, sizeof(float) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(dYn, Yn, sizeof(float) * 4 * n, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(ctx, launch_mlp_grad(dX, dYn, n, s->F, partial, lossp, dg, s->split, ctx->stream));
+    HIPCHK(ctx, launch_mlp_grad(dX, dYn, n, s->F, partial, lossp, dg, s->fmt != kActBf16, ctx->stream));
     std::vector<float> lp((size_t)(n / 64));
     HIPCHK(ctx, hipMemcpyAsync(grad, dg, sizeof(float) * kNParam, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(lp.data(), lossp, sizeof(float) * (n / 64), hipMemcpyDeviceToHost, ctx->stream));
@@ -549,7 +553,7 @@ int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out) {
     HIPCHK(ctx, sc.get(&dO, 4 * (size_t)n));
     HIPCHK(ctx, hipMemcpyAsync(dTc, Tc, sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, launch_mlp_inputs(dTc, n, s->norm.t_mid, s->norm.t_half, dX, ctx->stream));
-    HIPCHK(ctx, launch_mlp_eval(dX, n, s->F, dO, s->split, ctx->stream));
+    HIPCHK(ctx, launch_mlp_eval(dX, n, s->F, dO, s->fmt, ctx->stream));
     std::vector<float> O((size_t)(4 * n));
     std::vector<float> Xh((size_t)(2 * n));
     HIPCHK(ctx, hipMemcpyAsync(O.data(), dO, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));

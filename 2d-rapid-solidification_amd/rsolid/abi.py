@@ -18,6 +18,7 @@ RS_PREC_FP64 = 0
 RS_PREC_MIXED = 1
 RS_PREC_SURROGATE = 2
 
+RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16 = 0, 1, 2
 RS_SURR_NPARAM = 4612
 RS_SURR_NMETA = 14
 RS_SURR_NREPORT = 12
@@ -89,7 +90,7 @@ class RsSurrogateOpts(C.Structure):
     _fields_ = [
         ("n_uniform", C.c_int64), ("n_near", C.c_int64), ("seed", C.c_uint64),
         ("epochs", _I), ("batch", _I), ("T_lo", _D), ("T_hi", _D), ("lr", _D),
-        ("lr_final", _D), ("residual_tol", _D), ("n_curve", _I), ("split_act", _I),
+        ("lr_final", _D), ("residual_tol", _D), ("n_curve", _I), ("act_format", _I),
     ]
 
     def as_dict(self) -> dict:

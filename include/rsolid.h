@@ -298,9 +298,10 @@ int rs_load_state(rs_ctx* ctx, const char* path);
  * terms, which stay exact in FP64. It is trained on the GPU (BF16 MFMA,
  * FP32 accumulate, Adam on FP32 master weights) from samples the exact
  * routine generates, and evaluated inside the RS_PREC_SURROGATE step
- * kernel. A trust table of 64 x 64 bins over the normalised (T, x_Nb)
- * domain, built by comparing the network with the exact routine on a
- * 513 x 513 grid, marks the bins whose nondimensional residual
+ * kernel (MFMA in the activation format act_format). A trust table of
+ * 64 x 64 bins over the normalised (T, x_Nb) domain, built by comparing
+ * the network (in that format) with the exact routine on a 513 x 513
+ * grid, marks the bins whose nondimensional residual
  *   max(|d(f_L - f_S)| / (2 hh v0), |d mu_L| / (h0 v0), |d mu_S| / (h0 v0))
  * stays within residual_tol; cells outside the domain or in an untrusted
  * bin fall back to the exact routine. */
@@ -316,13 +317,22 @@ typedef struct rs_surrogate_opts {
     double lr, lr_final;   /* Adam step size, cosine-annealed to lr_final     */
     double residual_tol;   /* trust bound on the residual above (1e-4)        */
     int32_t n_curve;       /* points of the liquidus/solidus tables           */
-    int32_t split_act;     /* 1: hidden activations enter the next product as
-                              two BF16 parts (hi + lo), 0: one BF16 part     */
+    int32_t act_format;    /* inference format of the hidden activations,
+                              RS_ACT_* (RS_ACT_FP16)                          */
 } rs_surrogate_opts;
+
+/* Hidden activations are computed in FP32 and rounded once where they enter
+ * the next product. Training always runs RS_ACT_BF16 (RS_ACT_BF16 inference)
+ * or RS_ACT_BF16_SPLIT (the other two); weights are BF16 values in all. */
+enum {
+    RS_ACT_BF16 = 0,       /* one BF16 part (8 significant bits), BF16 MFMA   */
+    RS_ACT_BF16_SPLIT = 1, /* two BF16 parts hi + lo (~16 bits), 2 MFMAs      */
+    RS_ACT_FP16 = 2        /* FP16 (11 bits), FP16 MFMA for layers 2 and 3    */
+};
 
 #define RS_SURR_NPARAM 4612  /* W1[64][2] b1[64] W2[64][64] b2[64] W3[4][64] b3[4] */
 #define RS_SURR_NMETA 14     /* t_mid, t_half, y_mean[4], y_std[4], T_lo, T_hi, tol,
-                                split_act */
+                                act_format */
 #define RS_SURR_NREPORT 12
 
 int rs_surrogate_defaults(rs_surrogate_opts* o);
@@ -336,7 +346,7 @@ int rs_surrogate_defaults(rs_surrogate_opts* o);
 int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, int32_t n);
 /* Current master weights (FP32, RS_SURR_NPARAM) and meta (RS_SURR_NMETA). */
 int rs_surrogate_get(rs_ctx* ctx, float* params, int32_t np, double* meta, int32_t nm);
-/* Installs weights and meta, repacks the BF16 fragments and rebuilds the
+/* Installs weights and meta, repacks the weight fragments and rebuilds the
  * trust table. */
 int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double* meta, int32_t nm);
 /* Gradient of the mean squared error over n rows (n a multiple of 64,

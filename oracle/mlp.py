@@ -23,6 +23,11 @@ split=True (with quantized) states the split-activation kernels: each
 activation enters the next product as hi + lo, hi = bf16(h), lo = bf16(h - hi)
 (h in FP32), while the weight gradients of W2 and W3 take hi only.
 
+fp16=True (with quantized; inference only) states the FP16-activation
+kernels: h1 and h2 rounded to FP16 (nearest even) instead of BF16, and W2,
+W3 enter as their BF16 values held in FP16 (exact unless below 2^-14, where
+FP16 subnormals round them); layer 1 is as above.
+
 quantized=False is the exact float64 network; its gradient is pinned against
 central finite differences in the tests.
 """
@@ -45,6 +50,12 @@ def bf16(x) -> np.ndarray:
     u = np.asarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
     u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def fp16(x) -> np.ndarray:
+    """Round FP32 values to FP16 (nearest, ties to even, with subnormals);
+    returns float64."""
+    return np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float64)
 
 
 def split3(x) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -80,12 +91,14 @@ def _f32(x):
     return np.asarray(x, dtype=np.float32).astype(np.float64)
 
 
-def _acts(z: np.ndarray, quantized: bool, split: bool):
+def _acts(z: np.ndarray, quantized: bool, split: bool, half: bool = False):
     """(value entering the next product, value the weight gradient takes)."""
     h = act(z)
     if not quantized:
         return h, h
     h32 = np.asarray(h, dtype=np.float32)
+    if half:
+        return fp16(h32), fp16(h32)
     hi = bf16(h32)
     if not split:
         return hi, hi
@@ -93,11 +106,14 @@ def _acts(z: np.ndarray, quantized: bool, split: bool):
     return hi + lo, hi
 
 
-def forward(P: np.ndarray, X: np.ndarray, quantized: bool = True, split: bool = False) -> dict:
+def forward(P: np.ndarray, X: np.ndarray, quantized: bool = True, split: bool = False,
+            fp16_acts: bool = False) -> dict:
     """X: n x 2 inputs (a, b). Returns the intermediates and y (n x 4)."""
     W1, b1, W2, b2, W3, b3 = unpack(P)
     if quantized:
         W1q, W2q, W3q = bf16(W1), bf16(W2), bf16(W3)
+        if fp16_acts:
+            W2q, W3q = fp16(W2q), fp16(W3q)
         b1, b2, b3 = _f32(b1), _f32(b2), _f32(b3)
         pa = split3(X[:, 0])
         pb = split3(X[:, 1])
@@ -109,9 +125,9 @@ def forward(P: np.ndarray, X: np.ndarray, quantized: bool = True, split: bool = 
         xb = np.asarray(X[:, 1], dtype=np.float64)
         pa = pb = None
     z1 = np.outer(xa, W1q[:, 0]) + np.outer(xb, W1q[:, 1]) + b1
-    h1, g1 = _acts(z1, quantized, split)
+    h1, g1 = _acts(z1, quantized, split, fp16_acts)
     z2 = h1 @ W2q.T + b2
-    h2, g2 = _acts(z2, quantized, split)
+    h2, g2 = _acts(z2, quantized, split, fp16_acts)
     y = h2 @ W3q.T + b3
     return {"z1": z1, "h1": h1, "z2": z2, "h2": h2, "g1": g1, "g2": g2, "y": y, "pa": pa, "pb": pb,
             "xa": xa, "xb": xb, "W1q": W1q, "W2q": W2q, "W3q": W3q}

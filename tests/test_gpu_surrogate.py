@@ -6,8 +6,9 @@ oracle.
 Stated tolerances:
   gradient (random weights, 4096 rows): relative L2 error <= 2e-3 overall
     and <= 1e-2 per parameter block; loss relative error <= 1e-5
-  forward: |y_gpu - y_oracle| <= 1e-3 max|y| (BF16 activations may round
-    the other way for rare elements; accumulation is FP32 vs float64)
+  forward, each activation format (BF16, split BF16, FP16):
+    |y_gpu - y_oracle| <= 1e-3 max|y| (activations may round the other way
+    for rare elements; accumulation is FP32 vs float64)
   trusted outputs vs exact CALPHAD: residual <= residual_tol (1e-4, the
     nondimensional bound the trust table enforces) + 2e-5 margin for points
     between the validation grid nodes
@@ -26,12 +27,13 @@ import numpy as np
 import pytest
 
 from oracle import mlp
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_PREC_SURROGATE
+from rsolid.abi import (RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_FIELD_C, RS_FIELD_PSI,
+                        RS_PREC_SURROGATE)
 
 pytestmark = pytest.mark.gpu
 
-def meta_test(split):
-    return np.array([1930.0, 20.0, 0, 0, 0, 0, 1, 1, 1, 1, 1910.0, 1950.0, 1e-4, float(split)])
+def meta_test(fmt):
+    return np.array([1930.0, 20.0, 0, 0, 0, 0, 1, 1, 1, 1, 1910.0, 1950.0, 1e-4, float(fmt)])
 BLOCKS = {"W1": (mlp.OFF_W1, mlp.OFF_B1), "b1": (mlp.OFF_B1, mlp.OFF_W2),
           "W2": (mlp.OFF_W2, mlp.OFF_B2), "b2": (mlp.OFF_B2, mlp.OFF_W3),
           "W3": (mlp.OFF_W3, mlp.OFF_B3), "b3": (mlp.OFF_B3, mlp.NPARAM)}
@@ -61,7 +63,7 @@ def test_gradient_matches_oracle(rs, split):
     p = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
     ctx = rs.Context(p)
     P = mlp.init_params(7)
-    ctx.surrogate_set(P, meta_test(split))
+    ctx.surrogate_set(P, meta_test(RS_ACT_BF16_SPLIT if split else RS_ACT_BF16))
     rng = np.random.default_rng(11)
     X = rng.uniform(-1, 1, (4096, 2)).astype(np.float32)
     Yn = rng.normal(size=(4096, 4)).astype(np.float32)
@@ -80,17 +82,17 @@ def test_gradient_matches_oracle(rs, split):
     ctx.close()
 
 
-@pytest.mark.parametrize("split", [False, True])
-def test_forward_matches_oracle(rs, split):
+@pytest.mark.parametrize("fmt", [RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16])
+def test_forward_matches_oracle(rs, fmt):
     p = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
     ctx = rs.Context(p)
     P = mlp.init_params(9)
-    ctx.surrogate_set(P, meta_test(split))
+    ctx.surrogate_set(P, meta_test(fmt))
     rng = np.random.default_rng(12)
     n = 5000
     T = rng.uniform(1910.0, 1950.0, n)
     c = 1.0 - rng.uniform(0.0, 0.4, n)
-    say("forward check", n)
+    say("forward check", n, "format", fmt)
     out = ctx.surrogate_eval(T, c)
     RR = p.RR
     idl = RR * T * (c * np.log(c) + (1 - c) * np.log(1 - c))
@@ -98,7 +100,8 @@ def test_forward_matches_oracle(rs, split):
     y = np.stack([out[:, 0] - out[:, 1], out[:, 1] - idl, out[:, 2] - lr, out[:, 3] - out[:, 2]],
                  axis=1)
     X = np.stack([(T - 1930.0) / 20.0, ((1 - c) - 0.2) / 0.2], axis=1).astype(np.float32)
-    yo = mlp.forward(P, X, quantized=True, split=split)["y"]
+    yo = mlp.forward(P, X, quantized=True, split=fmt == RS_ACT_BF16_SPLIT,
+                     fp16_acts=fmt == RS_ACT_FP16)["y"]
     err = np.max(np.abs(y - yo))
     say("forward max err", err, "scale", np.max(np.abs(yo)))
     assert err <= 1e-3 * np.max(np.abs(yo))
@@ -150,11 +153,15 @@ def test_surrogate_step_vs_fp64_and_ablation(rs, oracle_mod, trained):
     st = oracle_mod.State(pd)
     st.init_config(7)
     ps = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
+    assert meta[13] == RS_ACT_FP16  # the default inference format
+    meta_split = meta.copy()
+    meta_split[13] = RS_ACT_BF16_SPLIT
     runs = {}
-    for name, weights in (("trained", P), ("untrained", mlp.init_params(3))):
+    for name, weights, m in (("trained", P, meta), ("trained-split", P, meta_split),
+                             ("untrained", mlp.init_params(3), meta)):
         ctx = rs.Context(ps)
         ctx.init_config(7)
-        ctx.surrogate_set(weights, meta)
+        ctx.surrogate_set(weights, m)
         runs[name] = ctx
     # tip positions after 500 and 2000 steps
     x0 = {}
@@ -185,7 +192,7 @@ def test_surrogate_step_vs_fp64_and_ablation(rs, oracle_mod, trained):
         assert ec <= 2e-3
         say(name, "oracle drift", dmo)
         assert abs(dm) <= 1e-10 + 1e-2 * abs(dmo)
-        if name == "trained":
+        if name.startswith("trained"):
             assert stats["fallback_frac"] < 0.5
         else:
             assert stats["fallback_frac"] > 0.99

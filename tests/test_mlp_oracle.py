@@ -1,7 +1,8 @@
 """Pins of the surrogate network's CPU restatement (oracle/mlp.py).
 
 The network has no counterpart in the reference, so it is pinned by facts
-of the arithmetic itself: BF16 rounding of known values, the exactness of
+of the arithmetic itself: BF16 and FP16 rounding of known values, the
+ordering of the activation formats' errors, the exactness of
 the three-part input split, closed forms of the activation, and the
 analytic gradient of the exact network against central finite differences.
 """
@@ -22,6 +23,49 @@ def test_bf16_rounding_known_values():
     assert got[4] == -2.5
     # 3e-3 = 1.536 * 2^-9: 7 mantissa bits keep 1.5390625 * 2^-9
     assert got[5] == pytest.approx(1.5390625 * 2.0**-9, rel=0, abs=0)
+
+
+def test_fp16_rounding_known_values():
+    # IEEE binary16: 10 stored mantissa bits, ties to even, subnormals down
+    # to 2^-24, largest finite 65504
+    x = np.array([1.0 + 2.0**-11, 1.0 + 3 * 2.0**-11, 1.0 / 3.0, 65504.0, 2.0**-24, 2.0**-25,
+                  3 * 2.0**-25, 2.0**-14 * (1 + 2.0**-10), -0.1], dtype=np.float32)
+    got = mlp.fp16(x)
+    assert got[0] == 1.0                         # tie -> even
+    assert got[1] == 1.0 + 2.0**-9               # tie -> even (upward)
+    assert got[2] == 0.333251953125              # 1365 * 2^-12
+    assert got[3] == 65504.0
+    assert got[4] == 2.0**-24                    # smallest subnormal
+    assert got[5] == 0.0                         # half of it: tie -> even (0)
+    assert got[6] == 2.0**-23                    # 1.5 subnormal steps: tie -> even (2)
+    assert got[7] == 2.0**-14 * (1 + 2.0**-10)   # normal, exact
+    assert got[8] == -0.0999755859375            # -1638 * 2^-14
+    # BF16 values of normal FP16 magnitude are exact in FP16
+    rng = np.random.default_rng(4)
+    w = mlp.bf16(rng.uniform(-2, 2, 10000).astype(np.float32))
+    w = w[np.abs(w) >= 2.0**-14]
+    assert np.array_equal(mlp.fp16(w), w)
+
+
+def test_fp16_network_between_bf16_and_split():
+    """FP16 activations carry 11 bits: their error against the FP32-activation
+    network lies between the BF16 (8 bits) and the split BF16 (~16 bits)
+    formats'."""
+    rng = np.random.default_rng(6)
+    P = mlp.init_params(2)
+    X = rng.uniform(-1, 1, (4096, 2)).astype(np.float32)
+    W1, b1, W2, b2, W3, b3 = mlp.unpack(P)
+    # reference: the quantized weights and inputs with unrounded activations
+    Pq = P.astype(np.float64).copy()
+    for off, end in ((mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_W3, mlp.OFF_B3)):
+        Pq[off:end] = mlp.bf16(P[off:end])
+    yr = mlp.forward(Pq, X, quantized=False)["y"]
+    e = {}
+    for name, kw in (("bf16", {}), ("split", {"split": True}), ("fp16", {"fp16_acts": True})):
+        e[name] = np.max(np.abs(mlp.forward(P, X, True, **kw)["y"] - yr))
+    assert e["split"] < e["fp16"] < e["bf16"]
+    # each extra mantissa bit roughly halves the error: 3 bits -> ~8x
+    assert 3.0 < e["bf16"] / e["fp16"] < 20.0
 
 
 def test_split3_carries_fp32_exactly():
