@@ -56,6 +56,8 @@ int check_params(const rs_params& p, std::string* why) {
         // the mixed stepper indexes the coarse-T rows with integer arithmetic
         const double dxT = p.dx * p.ratio_dxT_dx;
         if (fabs(dxT - (double)lround(dxT)) > 1e-12) { *why = "mixed path needs dx*ratio_dxT_dx integral"; return RS_ERR_ARG; }
+        // its raw buffer views address an FP64 field with 32-bit byte offsets
+        if ((int64_t)p.Nx * p.Ny * 8 >= (int64_t)1 << 32) { *why = "mixed path needs Nx*Ny < 2^29 cells"; return RS_ERR_ARG; }
     }
     return RS_OK;
 }
@@ -571,7 +573,9 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             rc = do_pull_back(ctx, it - p.i_tip_target);
             if (rc) return rc;
         }
-        if (ctx->timing) {
+        // event pairs around the kernels of every timing-th step
+        const bool timed = ctx->timing > 0 && ctx->timing_ctr++ % ctx->timing == 0;
+        if (timed) {
             int rc = ev_record(ctx, 0);
             if (rc) return rc;
         }
@@ -602,13 +606,13 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
                                             p.Nx, p.Ny, p.dx, k.dt, p.at_coef, p.k_part, vdn,
                                             ctx->stream));
         }
-        if (ctx->timing) {
+        if (timed) {
             int rc = ev_record(ctx, 0);
             if (rc) return rc;
         }
         if (p.ifTFC) {
             TempArgs t = temp_args(ctx, step);
-            if (ctx->timing) {
+            if (timed) {
                 int rc = ev_record(ctx, 1);
                 if (rc) return rc;
             }
@@ -619,7 +623,7 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
                 HIPCHK(ctx, launch_temperature(t, ctx->stream));
                 ctx->tcur ^= 1;
             }
-            if (ctx->timing) {
+            if (timed) {
                 int rc = ev_record(ctx, 1);
                 if (rc) return rc;
             }
@@ -733,10 +737,11 @@ int rs_sync(rs_ctx* ctx) {
     return RS_OK;
 }
 
-int rs_timing_enable(rs_ctx* ctx, int32_t on) {
-    if (!ctx) return RS_ERR_ARG;
+int rs_timing_enable(rs_ctx* ctx, int32_t every) {
+    if (!ctx || every < 0) return RS_ERR_ARG;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->timing = on != 0;
+    ctx->timing = every;
+    ctx->timing_ctr = 0;
     for (int q = 0; q < 3; ++q) ctx->nev[q] = 0;
     return RS_OK;
 }

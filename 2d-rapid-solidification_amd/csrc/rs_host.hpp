@@ -69,9 +69,10 @@ struct rs_ctx {
     int cur = 0, tcur = 0;
     int band_h = 40, wpb = 4;
     std::string err;
-    // HIP-event timing of the fused step kernel [0], coarse T kernel [1] and
-    // the whole step [2]
-    bool timing = false;
+    // HIP-event timing of the fused step kernel [0] and the coarse T kernel
+    // [1] on every timing-th step (0: off)
+    int timing = 0;
+    int64_t timing_ctr = 0;
     std::vector<hipEvent_t> ev[3];
     int nev[3] = {0, 0, 0};
 };

@@ -66,6 +66,12 @@ __device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int 
         return;
     }
     if (jT == 0 || jT == a.NyT - 1) return;
+    const double* T = a.T;
+    const double T00 = T[jT * P + iT];
+    const double T10 = T[jT * P + iT + 1];
+    const double Tm0 = T[jT * P + iT - 1];
+    const double T01 = T[(jT + 1) * P + iT];
+    const double T0m = T[(jT - 1) * P + iT];
     const int r = a.ratio;
     const int iT1 = a.ile / r;
     const int iTn = (a.ile + a.Nx - 3) / r;
@@ -77,15 +83,19 @@ __device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int 
         if (in > a.Nx - 2) in = a.Nx - 2;
         const double* row = a.lh + (int64_t)(jT - 1) * a.Nx;
         double integral = 0.0;
-        for (int i = i0; i <= in; ++i) integral += row[i];
+        if (in - i0 == 9) {
+            // a full block of 10 columns (the library's ratio): all loads in
+            // flight before the in-order sum
+            double v[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) v[q] = row[i0 + q];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) integral += v[q];
+        } else {
+            for (int i = i0; i <= in; ++i) integral += row[i];
+        }
         dphidt = integral * 0.5;
     }
-    const double* T = a.T;
-    const double T00 = T[jT * P + iT];
-    const double T10 = T[jT * P + iT + 1];
-    const double Tm0 = T[jT * P + iT - 1];
-    const double T01 = T[(jT + 1) * P + iT];
-    const double T0m = T[(jT - 1) * P + iT];
     const double lap = (T10 + Tm0 + T01 + T0m - 4.0 * T00) / (a.dxT * a.dxT);
     const double dTdt = a.vadv * (T10 - Tm0) / (2.0 * a.dxT) + lap * a.DT / a.Gmu;
     const double tn = T00 + a.dtT * dTdt + a.Lcp * dphidt / (a.ratio_dt_dtT * (r * r));

@@ -85,31 +85,76 @@ __device__ __forceinline__ double ratio_c(double c, double om) {
 // (ratio_dxT_dx, fun.cu:109) is fixed at 10 by the library.
 constexpr int kRatio = 10;
 
-struct RowT {
-    const float* r0;
-    const float* r1;
-    float y;
+// Fine T' of a lane's column (fun.cu:1580 bilinear form): T' = fmaf(A, y, B)
+// with A = fmaf(a11, x, a01), B = fmaf(a10, x, f00). x and the four coarse
+// values are fixed for a lane within one coarse row (10 fine rows), so A and
+// B are cached and each fine row costs one FMA; the operations, hence the
+// values, are those of the direct form.
+struct TCache {
+    int jT;        // coarse row of the cached terms (-1: none)
+    float A, B;
+    float x;
+    uint32_t off;  // byte offset of the lane's coarse column iT in a row
 };
 
-__device__ __forceinline__ RowT row_t(const StepArgsMixed& a, int j) {
-    RowT rt;
-    const int g = j - 1 + a.dxTi;
-    const int jT = g / kRatio;
-    rt.r0 = a.T32 + jT * a.NxT_all;
-    rt.r1 = rt.r0 + a.NxT_all;
-    rt.y = (float)(g - kRatio * jT) * 0.1f;
-    return rt;
+__device__ __forceinline__ TCache tcache_init(const StepArgsMixed& a, int i) {
+    TCache tc;
+    tc.jT = -1;
+    tc.A = tc.B = 0.f;
+    const int g = i - 1 + a.ile;
+    const int iT = g / kRatio;
+    tc.x = (float)(g - kRatio * iT) * 0.1f;
+    tc.off = (uint32_t)iT * 4u;
+    return tc;
 }
 
-__device__ __forceinline__ float temp_m(const StepArgsMixed& a, const RowT& rt, int i) {
+template <typename T>
+__device__ __forceinline__ T ld_at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Raw buffer views of the row-major fields: the descriptor (base, size)
+// lives in scalar registers, the row offset is a 32-bit scalar and the lane
+// offset a 32-bit vector register, so an access costs no 64-bit address
+// arithmetic on the vector ALU. Word 3 of the descriptor is the gfx9
+// format word (32-bit data format, no swizzle). Fields must stay below
+// 4 GiB (checked at context creation).
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+__device__ __forceinline__ Rsrc rsrc_of(const void* p, uint64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0,
+                                             (int)(uint32_t)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_ldf(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double buf_ldd(Rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_stf(Rsrc r, uint32_t voff, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_std(Rsrc r, uint32_t voff, uint32_t soff, double v) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
+}
+
+// T' of fine row j (wavefront-uniform) in the lane's column i
+__device__ __forceinline__ float temp_c(const StepArgsMixed& a, TCache& tc, int j, int i) {
     if (a.tfc) {
-        const int g = i - 1 + a.ile;
-        const int iT = g / kRatio;
-        const float x = (float)(g - kRatio * iT) * 0.1f;
-        const float f00 = rt.r0[iT], f10 = rt.r0[iT + 1];
-        const float f01 = rt.r1[iT], f11 = rt.r1[iT + 1];
-        const float a10 = f10 - f00, a01 = f01 - f00, a11 = (f11 - f10) - (f01 - f00);
-        return fmaf(fmaf(a11, x, a01), rt.y, fmaf(a10, x, f00));
+        const int g = j - 1 + a.dxTi;
+        const int jT = g / kRatio;
+        if (jT != tc.jT) {
+            tc.jT = jT;
+            const float* r0 = a.T32 + (size_t)jT * a.NxT_all;
+            const float* r1 = r0 + a.NxT_all;
+            const float f00 = ld_at(r0, tc.off), f10 = ld_at(r0, tc.off + 4u);
+            const float f01 = ld_at(r1, tc.off), f11 = ld_at(r1, tc.off + 4u);
+            const float a10 = f10 - f00, a01 = f01 - f00, a11 = (f11 - f10) - (f01 - f00);
+            tc.A = fmaf(a11, tc.x, a01);
+            tc.B = fmaf(a10, tc.x, f00);
+        }
+        const float y = (float)(g - kRatio * jT) * 0.1f;
+        return fmaf(tc.A, y, tc.B);
     }
     return (float)fma(a.fta_b, (double)i, a.fta_a);
 }
@@ -250,7 +295,7 @@ struct CellM {
 
 template <int SURR>
 __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const double* lntab,
-                                             const SurrLds& sl, const RowT& rt, float psi,
+                                             const SurrLds& sl, float Tp, float psi,
                                              double c, int i, int j, bool real, unsigned& nfb,
                                              unsigned& nall) {
     CellM q;
@@ -259,7 +304,6 @@ __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const doubl
     q.phi = fmaxf(psi, -1.0f);
     const float cf = (float)c;
     q.S = qq<float>(a.AA, a.q_solid, psi) * (cf * (1.0f - cf));
-    const float Tp = temp_m(a, rt, i);
     const float h = 0.5f * (1.0f + gg<float>(psi));
     if constexpr (SURR != 0)
         thermo_s<SURR - 1>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb, nall);
@@ -336,9 +380,10 @@ struct Slot {
 
 template <int SURR>
 __device__ __forceinline__ void fill_slot(const StepArgsMixed& a, const double* lntab,
-                                          const SurrLds& sl, int j, float psi, double c, int ic,
-                                          bool real, Slot& s, unsigned& nfb, unsigned& nall) {
-    s.q = make_cell_m<SURR>(a, lntab, sl, row_t(a, j), psi, c, ic, j, real, nfb, nall);
+                                          const SurrLds& sl, TCache& tc, int j, float psi,
+                                          double c, int ic, bool real, Slot& s, unsigned& nfb,
+                                          unsigned& nall) {
+    s.q = make_cell_m<SURR>(a, lntab, sl, temp_c(a, tc, j, ic), psi, c, ic, j, real, nfb, nall);
     s.phL = shup(s.q.phi);
     s.phR = shdn(s.q.phi);
     s.SR = shdn(s.q.S);
@@ -385,11 +430,19 @@ __device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
 
 // One wavefront's work: column tile tile_x (62 output columns), band `band`
 // (rows 1 + band*band_h ...), marching down the band with a 3-row window.
-template <bool IDENT, int SURR>
+// BUF: field rows through raw buffer views (no 64-bit address arithmetic on
+// the vector ALU); the persistent batch kernel, whose run state already
+// occupies the scalar registers, addresses them with plain pointers. The
+// two address the same elements.
+template <bool IDENT, int SURR, bool BUF = true>
 __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int tile_x, int band, unsigned& nfb,
                                           unsigned& nall) {
     const int lane = threadIdx.x;
+    // wavefront-uniform (blockDim.x is the wavefront): row addresses then
+    // live in scalar registers and each access adds a 32-bit lane offset
+    band = __builtin_amdgcn_readfirstlane(band);
+    tile_x = __builtin_amdgcn_readfirstlane(tile_x);
     if (band >= a.nbands) return;
     const int i = tile_x * kTileOut + lane;
     const int ic = i < a.Nx ? i : a.Nx - 1;
@@ -397,20 +450,35 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
     const int jb = 1 + band * a.band_h;
     const int je = min(jb + a.band_h - 1, a.Ny - 2);
     const int Nx = a.Nx;
+    const uint64_t cells = (uint64_t)Nx * a.Ny;
+    const uint32_t o4 = (uint32_t)ic * 4u, o8 = (uint32_t)ic * 8u;
+    const uint32_t w4 = (uint32_t)i * 4u, w8 = (uint32_t)i * 8u;
+    auto psi_row = [&](int r) {
+        if constexpr (BUF)
+            return buf_ldf(rsrc_of(a.psi, 4 * cells), o4, (uint32_t)(r * Nx) * 4u);
+        else
+            return a.psi[(size_t)r * Nx + ic];
+    };
+    auto c_row = [&](int r) {
+        if constexpr (BUF)
+            return buf_ldd(rsrc_of(a.c, 8 * cells), o8, (uint32_t)(r * Nx) * 8u);
+        else
+            return a.c[(size_t)r * Nx + ic];
+    };
 
     Slot s0, s1, s2;
+    TCache tc = tcache_init(a, ic);
     const bool real = i < a.Nx;  // lanes past the last column duplicate it
-    fill_slot<SURR>(a, lntab, sl, jb - 1, a.psi[(jb - 1) * Nx + ic], a.c[(jb - 1) * Nx + ic], ic,
-                    real, s0, nfb, nall);
-    fill_slot<SURR>(a, lntab, sl, jb, a.psi[jb * Nx + ic], a.c[jb * Nx + ic], ic, real, s1, nfb,
+    fill_slot<SURR>(a, lntab, sl, tc, jb - 1, psi_row(jb - 1), c_row(jb - 1), ic, real, s0, nfb,
                     nall);
+    fill_slot<SURR>(a, lntab, sl, tc, jb, psi_row(jb), c_row(jb), ic, real, s1, nfb, nall);
     // vertex (i+1/2, jb-1/2) and the y-face flux (i, jb-1/2)
     float Vpm = quad4(s0.q.S, s0.SR, s1.q.S, s1.SR);
     const float Vmm = shup(Vpm);
     double Fy_prev = (double)quad4(s0.q.S, s1.q.S, Vpm, Vmm) * (s1.q.mu - s0.q.mu);
 
-    float nx_psi = a.psi[(jb + 1) * Nx + ic];
-    double nx_c = a.c[(jb + 1) * Nx + ic];
+    float nx_psi = psi_row(jb + 1);
+    double nx_c = c_row(jb + 1);
     float lh_acc = 0.0f;
 
     // output row j from rows M = j-1, Z = j and P = j+1 (filled here)
@@ -418,10 +486,10 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
         const float cur_psi = nx_psi;
         const double cur_c = nx_c;
         if (j + 2 <= je + 1) {
-            nx_psi = a.psi[(j + 2) * Nx + ic];
-            nx_c = a.c[(j + 2) * Nx + ic];
+            nx_psi = psi_row(j + 2);
+            nx_c = c_row(j + 2);
         }
-        fill_slot<SURR>(a, lntab, sl, j + 1, cur_psi, cur_c, ic, real, P, nfb, nall);
+        fill_slot<SURR>(a, lntab, sl, tc, j + 1, cur_psi, cur_c, ic, real, P, nfb, nall);
         const double muR_z = shdn(Z.q.mu);
         // vertex (i+1/2, j+1/2), its left neighbour, and the faces
         const float Vpp = quad4(Z.q.S, Z.SR, P.q.S, P.SR);
@@ -446,10 +514,15 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
             const double dc = a.K * ((Fx - Fxm) + (Fy - Fy_prev));
             const double cn = fmax(Z.q.c + dc, 1.0e-6);
 
-            float* prow = a.psi_n + j * Nx;
-            double* crow = a.c_n + j * Nx;
-            prow[i] = psn;
-            crow[i] = cn;
+            float* prow = a.psi_n + (size_t)j * Nx;
+            double* crow = a.c_n + (size_t)j * Nx;
+            if constexpr (BUF) {
+                buf_stf(rsrc_of(a.psi_n, 4 * cells), w4, (uint32_t)(j * Nx) * 4u, psn);
+                buf_std(rsrc_of(a.c_n, 8 * cells), w8, (uint32_t)(j * Nx) * 8u, cn);
+            } else {
+                prow[i] = psn;
+                crow[i] = cn;
+            }
             if (i == 1) {
                 prow[0] = psn;
                 crow[0] = cn;
@@ -474,7 +547,7 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
             if (a.lh_on && j <= a.lh_last_row) {
                 lh_acc += psn - Z.q.psi;
                 if (j % 10 == 0) {
-                    a.lh[(j / 10 - 1) * Nx + i] = (double)lh_acc;
+                    a.lh[(size_t)(j / 10 - 1) * Nx + i] = (double)lh_acc;
                     lh_acc = 0.0f;
                 }
             }
@@ -495,9 +568,10 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
 }
 
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
-// SURR - 1 (kActBf16, kActSplit, kActFp16). At least 4 wavefronts per SIMD (<= 128 VGPRs, no spills) for
-// the exact thermodynamics: the kernel is latency-bound below that. The
-// surrogate variants hold the MFMA operands and run at 2.
+// SURR - 1 (kActBf16, kActSplit, kActFp16). 4 wavefronts per SIMD (<= 128
+// VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
+// 4096^2 step takes 0.191 ms instead of 0.166 ms. The surrogate variants
+// hold the MFMA operands and run at 2 (3 spills 52 registers).
 template <bool IDENT, int SURR>
 __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
     __shared__ StepLds<SURR> L;
@@ -564,9 +638,12 @@ __device__ __forceinline__ int batch_run_of(const int* pre, int nruns, int q) {
 
 // Persistent: each workgroup loads the LDS tables once and walks the work
 // list (run, column tile, band group) with a fixed stride, so the assignment
-// and the results do not depend on timing.
+// and the results do not depend on timing. Exact thermodynamics at 3
+// wavefronts per SIMD: the per-item run state spills at 4 (64-run ensemble:
+// 3 waves 0.425 ms per step, 4 waves with 10 spilled registers 0.49-0.52 ms);
+// plain pointers measured faster than buffer views here (0.49-0.51 ms).
 template <bool IDENT, int SURR>
-__global__ __launch_bounds__(256, SURR ? 2 : 4) void batch_step_kernel(BatchLaunch L) {
+__global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaunch L) {
     __shared__ StepLds<SURR> lds;
     load_step_lds<SURR>(lds, L.runs[0].a.mlp, L.runs[0].a.trust,
                         threadIdx.y * blockDim.x + threadIdx.x, blockDim.x * blockDim.y);
@@ -591,7 +668,7 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void batch_step_kernel(BatchLaun
         const int tx = local % R.ntx;
         const int bg = local / R.ntx;
         unsigned nfb = 0, nall = 0;
-        step_tile<IDENT, SURR>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
+        step_tile<IDENT, SURR, false>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
                                nfb, nall);
         if constexpr (SURR != 0) {
             if (threadIdx.x == 0 && nall > 0) {

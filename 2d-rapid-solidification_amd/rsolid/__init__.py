@@ -288,8 +288,9 @@ class Context:
     def sync(self) -> None:
         self._chk(lib.rs_sync(self._h), "rs_sync")
 
-    def timing_enable(self, on: bool = True) -> None:
-        self._chk(lib.rs_timing_enable(self._h, 1 if on else 0), "rs_timing_enable")
+    def timing_enable(self, on: bool = True, every: int = 1) -> None:
+        """HIP-event timing of the kernels of every `every`-th step."""
+        self._chk(lib.rs_timing_enable(self._h, every if on else 0), "rs_timing_enable")
 
     def info(self) -> dict:
         v = (C.c_double * 6)()

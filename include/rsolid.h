@@ -240,10 +240,13 @@ int rs_solute_mass(rs_ctx* ctx, double* mass);              /* sum of c, interio
 int rs_tip(rs_ctx* ctx, rs_tip_row* tip);                       /* set_tip_output */
 int rs_sync(rs_ctx* ctx);
 
-/* Average duration (ms) of the named kernel over the last rs_step call,
- * measured with HIP events on the context stream. which: 0 = fused
- * phi/psi/c step kernel, 1 = coarse T kernel, 2 = whole step. */
-int rs_timing_enable(rs_ctx* ctx, int32_t on);
+/* Kernel timing with HIP events on the context stream: after
+ * rs_timing_enable(ctx, every) (every >= 1; 0 switches timing off) the
+ * kernels of every every-th step are bracketed by event pairs, and
+ * rs_timing returns their average duration (ms) and count since the enable.
+ * which: 0 = fused phi/psi/c step kernel (with the antitrapping pass when
+ * on), 1 = coarse T kernel(s). */
+int rs_timing_enable(rs_ctx* ctx, int32_t every);
 int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
 
 /* Measurement helper with no counterpart in the reference: launches a plain
