@@ -407,6 +407,8 @@ template <>
 struct StepLds<2> : StepLds<1> {};
 template <>
 struct StepLds<3> : StepLds<1> {};
+template <>
+struct StepLds<4> : StepLds<1> {};
 
 template <int SURR>
 __device__ __forceinline__ void load_step_lds(StepLds<SURR>& L, const MlpInfer* mlp,
@@ -431,9 +433,10 @@ __device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
 // One wavefront's work: column tile tile_x (62 output columns), band `band`
 // (rows 1 + band*band_h ...), marching down the band with a 3-row window.
 // BUF: field rows through raw buffer views (no 64-bit address arithmetic on
-// the vector ALU); the persistent batch kernel, whose run state already
-// occupies the scalar registers, addresses them with plain pointers. The
-// two address the same elements.
+// the vector ALU). The surrogate kernels and the persistent batch kernel,
+// whose scalar registers already hold the network or run state (the four
+// descriptors would add 16 and spill more), address them with plain
+// pointers: measured 3% faster there. The two address the same elements.
 template <bool IDENT, int SURR, bool BUF = true>
 __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int tile_x, int band, unsigned& nfb,
@@ -568,7 +571,7 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
 }
 
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
-// SURR - 1 (kActBf16, kActSplit, kActFp16). 4 wavefronts per SIMD (<= 128
+// SURR - 1 (kActBf16, kActSplit, kActFp16, kActFp16Pk). 4 wavefronts per SIMD (<= 128
 // VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
 // 4096^2 step takes 0.191 ms instead of 0.166 ms. The surrogate variants
 // hold the MFMA operands and run at 2 (3 spills 52 registers).
@@ -579,7 +582,7 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
                         blockDim.x * blockDim.y);
     __syncthreads();
     unsigned nfb = 0, nall = 0;
-    step_tile<IDENT, SURR>(a, L.lntab, surr_lds<SURR>(L), blockIdx.x,
+    step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), blockIdx.x,
                            blockIdx.y * blockDim.y + threadIdx.y, nfb, nall);
     if constexpr (SURR != 0) {
         if (threadIdx.x == 0 && nall > 0) {
@@ -594,7 +597,12 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    if (surr && a.s_fmt == kActFp16) {
+    if (surr && a.s_fmt == kActFp16Pk) {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 4>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 4>), grid, block, 0, s, a);
+    } else if (surr && a.s_fmt == kActFp16) {
         if (ident)
             hipLaunchKernelGGL((step_mixed_kernel<true, 3>), grid, block, 0, s, a);
         else
@@ -727,7 +735,9 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
     case 4: e = batch_step_t<false, 2>(L, s); break;
     case 5: e = batch_step_t<true, 2>(L, s); break;
     case 6: e = batch_step_t<false, 3>(L, s); break;
-    default: e = batch_step_t<true, 3>(L, s); break;
+    case 7: e = batch_step_t<true, 3>(L, s); break;
+    case 8: e = batch_step_t<false, 4>(L, s); break;
+    default: e = batch_step_t<true, 4>(L, s); break;
     }
     if (e != hipSuccess) return e;
     if (L.cells > 0) {
@@ -757,7 +767,9 @@ int batch_step_resident(const BatchLaunch& L) {
     case 4: return resident_t<false, 2>(L.wpb);
     case 5: return resident_t<true, 2>(L.wpb);
     case 6: return resident_t<false, 3>(L.wpb);
-    default: return resident_t<true, 3>(L.wpb);
+    case 7: return resident_t<true, 3>(L.wpb);
+    case 8: return resident_t<false, 4>(L.wpb);
+    default: return resident_t<true, 4>(L.wpb);
     }
 }
 

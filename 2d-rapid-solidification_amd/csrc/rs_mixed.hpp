@@ -40,7 +40,7 @@ struct StepArgsMixed {
     unsigned long long* fb_count;
     float s_a_scale, s_a_off;  // a = T' s_a_scale + s_a_off
     float y_mean[4], y_std[4];
-    int s_fmt;                 // activation format (kActBf16, kActSplit, kActFp16)
+    int s_fmt;                 // activation format (kActBf16 ... kActFp16Pk)
 };
 
 // ident: the crystal orientation is the identity (the reference's only

@@ -365,7 +365,7 @@ template <int FMT>
 __global__ __launch_bounds__(256) void mlp_eval_kernel(const float* X, int64_t n,
                                                        const MlpFrags* Fg, float* out) {
     __shared__ MlpInfer F;
-    copy_words(&F, FMT == kActFp16 ? &Fg->h : &Fg->f, threadIdx.x, blockDim.x);
+    copy_words(&F, fp16_fmt(FMT) ? &Fg->h : &Fg->f, threadIdx.x, blockDim.x);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t r = ((int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * 64 + lane;
@@ -381,6 +381,8 @@ hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* 
     const dim3 grid((unsigned)((waves + 3) / 4));
     if (fmt == kActFp16)
         hipLaunchKernelGGL(mlp_eval_kernel<kActFp16>, grid, dim3(256), 0, s, X, n, F, out);
+    else if (fmt == kActFp16Pk)
+        hipLaunchKernelGGL(mlp_eval_kernel<kActFp16Pk>, grid, dim3(256), 0, s, X, n, F, out);
     else if (fmt == kActSplit)
         hipLaunchKernelGGL(mlp_eval_kernel<kActSplit>, grid, dim3(256), 0, s, X, n, F, out);
     else if (fmt == kActBf16)

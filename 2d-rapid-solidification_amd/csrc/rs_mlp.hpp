@@ -20,7 +20,10 @@
 //   kActSplit two BF16 parts, h = hi + lo, lo = bf16(h - hi) (~16 bits),
 //             two BF16 MFMAs per product;
 //   kActFp16  FP16 (11 bits; one v_cvt_pk_f16_f32 per pair), FP16 MFMA for
-//             layers 2 and 3 (weights: their BF16 values held in FP16).
+//             layers 2 and 3 (weights: their BF16 values held in FP16);
+//   kActFp16Pk as kActFp16, but the layer output is rounded to FP16 first
+//             and the activation computed in packed FP16 (v_pk_fma_f16 with
+//             the clamp, v_pk_mul_f16): 1.5 VALU per activation, 3 roundings.
 // Weights are BF16 values in every format, layer 1 is BF16 MFMA in every
 // format, and training runs the BF16 formats.
 //
@@ -75,7 +78,9 @@ constexpr int kNQ = kQB3 + 16;  // 6288
 // trust table: 64 x 64 bins over (a, b) in [-1, 1]^2, bit ia of word ib
 constexpr int kTrustN = 64;
 
-enum { kActBf16 = 0, kActSplit = 1, kActFp16 = 2 };
+enum { kActBf16 = 0, kActSplit = 1, kActFp16 = 2, kActFp16Pk = 3 };
+// formats whose layers 2 and 3 run FP16 MFMA on the MlpFrags::h set
+constexpr bool fp16_fmt(int fmt) { return fmt == kActFp16 || fmt == kActFp16Pk; }
 
 // weight fragments for inference (20.8 KB, staged in LDS); w1 is BF16, w2
 // and w3 BF16 (f) or FP16 (h) bits
@@ -179,6 +184,18 @@ __device__ __forceinline__ uint2 pack4f(f4 v) {
     else
         return pack4(v);
 }
+// hard SiLU of a pair in packed FP16: p = fp16(z), u = clamp(p/6 + 1/2)
+// (one FMA with the clamp, rounded), h = p u (rounded)
+__device__ __forceinline__ uint32_t act_pk2(float z0, float z1) {
+    const f2 v = {z0, z1};
+    const h2 p = __builtin_convertvector(v, h2);
+    const h2 c6 = {(_Float16)(1.0f / 6.0f), (_Float16)(1.0f / 6.0f)};
+    const h2 c05 = {(_Float16)0.5f, (_Float16)0.5f};
+    const h2 zero = {(_Float16)0.0f, (_Float16)0.0f}, one = {(_Float16)1.0f, (_Float16)1.0f};
+    h2 u = __builtin_elementwise_fma(p, c6, c05);
+    u = __builtin_elementwise_min(__builtin_elementwise_max(u, zero), one);
+    return __builtin_bit_cast(uint32_t, p * u);
+}
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -197,7 +214,7 @@ __device__ __forceinline__ f4 mfma32h(uint4 a, uint4 b, f4 c) {
 // the product of a hidden layer in format FMT
 template <int FMT>
 __device__ __forceinline__ f4 mfma32f(uint4 a, uint4 b, f4 c) {
-    if constexpr (FMT == kActFp16)
+    if constexpr (fp16_fmt(FMT))
         return mfma32h(a, b, c);
     else
         return mfma32(a, b, c);
@@ -265,9 +282,13 @@ template <int FMT>
 __device__ __forceinline__ void activate(const f4* z, Acts<FMT>& h) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        const f4 a = act4(z[mt]);
-        h.hi[mt] = pack4f<FMT>(a);
-        if constexpr (FMT == kActSplit) h.lo[mt] = pack4(a - unpack4(h.hi[mt]));
+        if constexpr (FMT == kActFp16Pk) {
+            h.hi[mt] = make_uint2(act_pk2(z[mt][0], z[mt][1]), act_pk2(z[mt][2], z[mt][3]));
+        } else {
+            const f4 a = act4(z[mt]);
+            h.hi[mt] = pack4f<FMT>(a);
+            if constexpr (FMT == kActSplit) h.lo[mt] = pack4(a - unpack4(h.hi[mt]));
+        }
     }
 }
 
@@ -302,19 +323,27 @@ __device__ __forceinline__ f4 layer3(const MlpInfer& F, const Acts<FMT>& h2, int
 
 // full forward of the wavefront's 64 samples; returns the lane's own
 // normalised outputs y0..y3 (F: the set of format FMT, MlpFrags::h for
-// kActFp16)
+// the FP16 formats)
 template <int FMT>
 __device__ __forceinline__ f4 forward(const MlpInfer& F, uint4 xin, int lane) {
     f4 y = {F.b3[0], F.b3[1], F.b3[2], F.b3[3]};
+    // two tiles in flight: one tile's activations fill the other's MFMA
+    // latency (the tiles are independent until layer 3, which accumulates
+    // in tile order)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        f4 z[4];
-        Acts<FMT> h;
-        layer1(F, xin, t, lane, z);
-        activate(z, h);
-        layer2(F, h, lane, z);
-        activate(z, h);
-        y = layer3(F, h, t, lane, y);
+    for (int t = 0; t < 4; t += 2) {
+        f4 za[4], zb[4];
+        Acts<FMT> ha, hb;
+        layer1(F, xin, t, lane, za);
+        layer1(F, xin, t + 1, lane, zb);
+        activate(za, ha);
+        activate(zb, hb);
+        layer2(F, ha, lane, za);
+        layer2(F, hb, lane, zb);
+        activate(za, ha);
+        activate(zb, hb);
+        y = layer3(F, ha, t, lane, y);
+        y = layer3(F, hb, t + 1, lane, y);
     }
     return y;
 }

@@ -215,7 +215,7 @@ bool surr_ready(const rs_ctx* ctx) { return ctx->surr && ctx->surr->ready; }
 
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
     const SurrState* s = ctx->surr;
-    a.mlp = s->fmt == kActFp16 ? &s->F->h : &s->F->f;
+    a.mlp = fp16_fmt(s->fmt) ? &s->F->h : &s->F->f;
     a.trust = s->trust;
     a.fb_count = s->count;
     // a = (T - t_mid)/t_half with T = Tref + T'
@@ -260,8 +260,8 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
         return fail(ctx, RS_ERR_ARG, "batch must be a multiple of 64 and at most the sample count");
     if (o->epochs < 0 || o->n_curve < 2 || !(o->residual_tol > 0.0) || !(o->lr > 0.0))
         return fail(ctx, RS_ERR_ARG, "bad surrogate options");
-    if (o->act_format < RS_ACT_BF16 || o->act_format > RS_ACT_FP16)
-        return fail(ctx, RS_ERR_ARG, "act_format must be RS_ACT_BF16, RS_ACT_BF16_SPLIT or RS_ACT_FP16");
+    if (o->act_format < RS_ACT_BF16 || o->act_format > RS_ACT_FP16_PACKED)
+        return fail(ctx, RS_ERR_ARG, "act_format must be one of RS_ACT_*");
     int rc = ensure_state(ctx);
     if (rc) return rc;
     SurrState* s = ctx->surr;
@@ -484,7 +484,8 @@ int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double*
     if (!ctx || !params || !meta) return RS_ERR_ARG;
     if (np != kNParam || nm < RS_SURR_NMETA) return fail(ctx, RS_ERR_ARG, "parameter or meta count");
     if (!(meta[1] > 0.0) || !(meta[12] > 0.0)) return fail(ctx, RS_ERR_ARG, "bad surrogate meta");
-    if (!(meta[13] == RS_ACT_BF16 || meta[13] == RS_ACT_BF16_SPLIT || meta[13] == RS_ACT_FP16))
+    if (!(meta[13] == RS_ACT_BF16 || meta[13] == RS_ACT_BF16_SPLIT || meta[13] == RS_ACT_FP16 ||
+          meta[13] == RS_ACT_FP16_PACKED))
         return fail(ctx, RS_ERR_ARG, "bad surrogate meta: act_format");
     for (int k = 0; k < 4; ++k)
         if (!(meta[6 + k] > 0.0)) return fail(ctx, RS_ERR_ARG, "bad surrogate meta");

@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
-    EXPORTED, RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_STATE, RS_FIELD_C,
+    EXPORTED, RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_STATE, RS_FIELD_C,
     RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE, RS_OK, RS_PREC_FP64, RS_PREC_MIXED,
     RS_PREC_SURROGATE, RS_SURR_NMETA, RS_SURR_NPARAM, RS_SURR_NREPORT, RsConstants, RsParams,
     RsSurrogateOpts, RsTipRow,

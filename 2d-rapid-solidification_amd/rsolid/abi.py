@@ -18,7 +18,7 @@ RS_PREC_FP64 = 0
 RS_PREC_MIXED = 1
 RS_PREC_SURROGATE = 2
 
-RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16 = 0, 1, 2
+RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED = 0, 1, 2, 3
 RS_SURR_NPARAM = 4612
 RS_SURR_NMETA = 14
 RS_SURR_NREPORT = 12

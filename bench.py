@@ -48,6 +48,8 @@ FLOPS_PER_CELL_MLP = 2 * (2 * 64 + 64 * 64 + 64 * 4)
 KERNEL = {"fp64": "step_fp64_kernel", "mixed": "step_mixed_kernel<IDENT=1,SURR=0>",
           "surrogate": "step_mixed_kernel<IDENT=1,SURR=3>"}
 TIMING_EVERY = 10
+# surrogate activation formats (RS_ACT_*)
+ACT_FORMATS = {"bf16": 0, "bf16-split": 1, "fp16": 2, "fp16-packed": 3}
 PROFILE = {"fp64": "step_fp64_kernel_4096.md", "mixed": "step_mixed_kernel_4096.md",
            "surrogate": "step_mixed_kernel_surrogate_4096.md"}
 
@@ -90,7 +92,7 @@ def cpu_baseline(grid: int, seconds: float):
 
 
 def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
-             train_epochs: int | None = None):
+             train_epochs: int | None = None, act_format: int | None = None):
     import rsolid
     prec = {"fp64": rsolid.RS_PREC_FP64, "mixed": rsolid.RS_PREC_MIXED,
             "surrogate": rsolid.RS_PREC_SURROGATE}[path]
@@ -99,8 +101,12 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     ctx.init_config(5)
     train = None
     if path == "surrogate":
-        train = (ctx.surrogate_train() if train_epochs is None
-                 else ctx.surrogate_train(epochs=train_epochs))
+        kw = {}
+        if train_epochs is not None:
+            kw["epochs"] = train_epochs
+        if act_format is not None:
+            kw["act_format"] = act_format
+        train = ctx.surrogate_train(**kw)
     ctx.step(warmup)
     ctx.sync()
     x0 = _tip_x(ctx)
@@ -136,7 +142,10 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     else:
         roof = dict(hbm)
         roof["bytes_per_cell"] = bpc
-    roof.update({"kernel": KERNEL[path], "kernel_avg_ms": round(k_ms, 5),
+    kname = KERNEL[path]
+    if path == "surrogate" and act_format is not None:
+        kname = f"step_mixed_kernel<IDENT=1,SURR={1 + act_format}>"
+    roof.update({"kernel": kname, "kernel_avg_ms": round(k_ms, 5),
                  "kernel_launches_timed": k_n, "kernel_launches": steps})
     return {
         "mlups": cells * steps / el / 1e6,
@@ -153,7 +162,7 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
 
 
 def run_ensemble(nruns: int, steps: int, warmup: int, seed: int, tune_steps: int,
-                 reference: bool = True, surrogate: bool = True):
+                 reference: bool = True, surrogate: bool = True, act_format: int | None = None):
     """Batched ensemble: nruns Ti-10at%Nb runs, grid sizes drawn by seed from
     {256^2, 512^2, 1024^2, 2048^2} (probabilities 0.4/0.3/0.2/0.1, so the
     counts are uneven), undercooling ~ U[25, 60] K and pulling speed
@@ -175,7 +184,10 @@ def run_ensemble(nruns: int, steps: int, warmup: int, seed: int, tune_steps: int
     b = rsolid.Batch(params)
     for run, s in zip(b.runs, seeds):
         run.init_config(s)
-    rep = b.surrogate_train() if surrogate else None
+    rep = None
+    if surrogate:
+        rep = (b.surrogate_train() if act_format is None
+               else b.surrogate_train(act_format=act_format))
     tuned = b.tune(steps=tune_steps)
     b.step(warmup)
     x0 = [_tip_x(r) for r in b.runs]
@@ -255,6 +267,8 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=None,
                     help="surrogate training epochs (default: the library's); profiler runs "
                          "that only need the step kernel's counters use 1")
+    ap.add_argument("--act-format", default="fp16", choices=sorted(ACT_FORMATS),
+                    help="surrogate inference format of the hidden activations")
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched-ensemble measurement (0: skip it)")
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -273,7 +287,8 @@ def main() -> int:
     res = {}
     for path in order:
         res[path] = run_path(path, a.grid, a.steps, a.warmup,
-                             a.calibrate_copy and path == a.path, a.train_epochs)
+                             a.calibrate_copy and path == a.path, a.train_epochs,
+                             ACT_FORMATS[a.act_format])
     head = res[a.path]
     dtype = {"fp64": "f64", "mixed": "f32+f64", "surrogate": "bf16/fp16+f32+f64"}[a.path]
     desc = {"fp64": "plain FP64 path",
@@ -325,7 +340,8 @@ def main() -> int:
                             "trusted_bin_frac": r["train"]["trusted_frac"],
                             "max_residual_trusted": r["train"]["max_res_trusted"]}
     if a.ensemble_runs > 0 and not a.only:
-        out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5)
+        out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
+                                       act_format=ACT_FORMATS[a.act_format])
         out["ensemble_fp32_exact_table"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024,
                                                         5, surrogate=False)
     if head["copy_ms"] is not None:

@@ -325,12 +325,19 @@ typedef struct rs_surrogate_opts {
 } rs_surrogate_opts;
 
 /* Hidden activations are computed in FP32 and rounded once where they enter
- * the next product. Training always runs RS_ACT_BF16 (RS_ACT_BF16 inference)
- * or RS_ACT_BF16_SPLIT (the other two); weights are BF16 values in all. */
+ * the next product (except RS_ACT_FP16_PACKED). Training always runs
+ * RS_ACT_BF16 (for RS_ACT_BF16 inference) or RS_ACT_BF16_SPLIT (the others);
+ * weights are BF16 values in all. */
 enum {
     RS_ACT_BF16 = 0,       /* one BF16 part (8 significant bits), BF16 MFMA   */
     RS_ACT_BF16_SPLIT = 1, /* two BF16 parts hi + lo (~16 bits), 2 MFMAs      */
-    RS_ACT_FP16 = 2        /* FP16 (11 bits), FP16 MFMA for layers 2 and 3    */
+    RS_ACT_FP16 = 2,       /* FP16 (11 bits), FP16 MFMA for layers 2 and 3    */
+    RS_ACT_FP16_PACKED = 3 /* as RS_ACT_FP16 with the activation itself in
+                              packed FP16 arithmetic (input, clamp-FMA and
+                              product each rounded to FP16): ~5% faster step,
+                              but its rounding noise lets points between the
+                              validation nodes of a trusted bin exceed
+                              residual_tol by up to ~25%                      */
 };
 
 #define RS_SURR_NPARAM 4612  /* W1[64][2] b1[64] W2[64][64] b2[64] W3[4][64] b3[4] */

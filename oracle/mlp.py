@@ -23,10 +23,13 @@ split=True (with quantized) states the split-activation kernels: each
 activation enters the next product as hi + lo, hi = bf16(h), lo = bf16(h - hi)
 (h in FP32), while the weight gradients of W2 and W3 take hi only.
 
-fp16=True (with quantized; inference only) states the FP16-activation
+fp16_acts=True (with quantized; inference only) states the FP16-activation
 kernels: h1 and h2 rounded to FP16 (nearest even) instead of BF16, and W2,
 W3 enter as their BF16 values held in FP16 (exact unless below 2^-14, where
-FP16 subnormals round them); layer 1 is as above.
+FP16 subnormals round them); layer 1 is as above. fp16_packed=True (with
+fp16_acts) computes the activation itself in FP16: p = fp16(z),
+u = clamp(fp16(p/6 + 1/2), 0, 1) with the 1/6 constant in FP16 and one
+rounding of the fused multiply-add, h = fp16(p u).
 
 quantized=False is the exact float64 network; its gradient is pinned against
 central finite differences in the tests.
@@ -91,8 +94,20 @@ def _f32(x):
     return np.asarray(x, dtype=np.float32).astype(np.float64)
 
 
-def _acts(z: np.ndarray, quantized: bool, split: bool, half: bool = False):
+def act_fp16_packed(z: np.ndarray) -> np.ndarray:
+    """Hard SiLU in FP16 arithmetic: each product of two FP16 values and the
+    sum with 1/2 are exact in float64, so one FP16 rounding per operation."""
+    p = fp16(np.asarray(z, dtype=np.float32))
+    u = np.clip(fp16(p * float(np.float16(1.0 / 6.0)) + 0.5), 0.0, 1.0)
+    return fp16(p * u)
+
+
+def _acts(z: np.ndarray, quantized: bool, split: bool, half: bool = False,
+          packed: bool = False):
     """(value entering the next product, value the weight gradient takes)."""
+    if quantized and half and packed:
+        h = act_fp16_packed(z)
+        return h, h
     h = act(z)
     if not quantized:
         return h, h
@@ -107,7 +122,7 @@ def _acts(z: np.ndarray, quantized: bool, split: bool, half: bool = False):
 
 
 def forward(P: np.ndarray, X: np.ndarray, quantized: bool = True, split: bool = False,
-            fp16_acts: bool = False) -> dict:
+            fp16_acts: bool = False, fp16_packed: bool = False) -> dict:
     """X: n x 2 inputs (a, b). Returns the intermediates and y (n x 4)."""
     W1, b1, W2, b2, W3, b3 = unpack(P)
     if quantized:
@@ -125,9 +140,9 @@ def forward(P: np.ndarray, X: np.ndarray, quantized: bool = True, split: bool = 
         xb = np.asarray(X[:, 1], dtype=np.float64)
         pa = pb = None
     z1 = np.outer(xa, W1q[:, 0]) + np.outer(xb, W1q[:, 1]) + b1
-    h1, g1 = _acts(z1, quantized, split, fp16_acts)
+    h1, g1 = _acts(z1, quantized, split, fp16_acts, fp16_packed)
     z2 = h1 @ W2q.T + b2
-    h2, g2 = _acts(z2, quantized, split, fp16_acts)
+    h2, g2 = _acts(z2, quantized, split, fp16_acts, fp16_packed)
     y = h2 @ W3q.T + b3
     return {"z1": z1, "h1": h1, "z2": z2, "h2": h2, "g1": g1, "g2": g2, "y": y, "pa": pa, "pb": pb,
             "xa": xa, "xb": xb, "W1q": W1q, "W2q": W2q, "W3q": W3q}

@@ -6,7 +6,7 @@ oracle.
 Stated tolerances:
   gradient (random weights, 4096 rows): relative L2 error <= 2e-3 overall
     and <= 1e-2 per parameter block; loss relative error <= 1e-5
-  forward, each activation format (BF16, split BF16, FP16):
+  forward, each activation format (BF16, split BF16, FP16, packed FP16):
     |y_gpu - y_oracle| <= 1e-3 max|y| (activations may round the other way
     for rare elements; accumulation is FP32 vs float64)
   trusted outputs vs exact CALPHAD: residual <= residual_tol (1e-4, the
@@ -27,8 +27,8 @@ import numpy as np
 import pytest
 
 from oracle import mlp
-from rsolid.abi import (RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_FIELD_C, RS_FIELD_PSI,
-                        RS_PREC_SURROGATE)
+from rsolid.abi import (RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED,
+                        RS_FIELD_C, RS_FIELD_PSI, RS_PREC_SURROGATE)
 
 pytestmark = pytest.mark.gpu
 
@@ -82,7 +82,7 @@ def test_gradient_matches_oracle(rs, split):
     ctx.close()
 
 
-@pytest.mark.parametrize("fmt", [RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16])
+@pytest.mark.parametrize("fmt", [RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED])
 def test_forward_matches_oracle(rs, fmt):
     p = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
     ctx = rs.Context(p)
@@ -101,7 +101,8 @@ def test_forward_matches_oracle(rs, fmt):
                  axis=1)
     X = np.stack([(T - 1930.0) / 20.0, ((1 - c) - 0.2) / 0.2], axis=1).astype(np.float32)
     yo = mlp.forward(P, X, quantized=True, split=fmt == RS_ACT_BF16_SPLIT,
-                     fp16_acts=fmt == RS_ACT_FP16)["y"]
+                     fp16_acts=fmt in (RS_ACT_FP16, RS_ACT_FP16_PACKED),
+                     fp16_packed=fmt == RS_ACT_FP16_PACKED)["y"]
     err = np.max(np.abs(y - yo))
     say("forward max err", err, "scale", np.max(np.abs(yo)))
     assert err <= 1e-3 * np.max(np.abs(yo))
@@ -156,8 +157,11 @@ def test_surrogate_step_vs_fp64_and_ablation(rs, oracle_mod, trained):
     assert meta[13] == RS_ACT_FP16  # the default inference format
     meta_split = meta.copy()
     meta_split[13] = RS_ACT_BF16_SPLIT
+    meta_packed = meta.copy()
+    meta_packed[13] = RS_ACT_FP16_PACKED
     runs = {}
     for name, weights, m in (("trained", P, meta), ("trained-split", P, meta_split),
+                             ("trained-packed", P, meta_packed),
                              ("untrained", mlp.init_params(3), meta)):
         ctx = rs.Context(ps)
         ctx.init_config(7)

@@ -47,6 +47,19 @@ def test_fp16_rounding_known_values():
     assert np.array_equal(mlp.fp16(w), w)
 
 
+def test_packed_fp16_activation_known_values():
+    # fp16(1/6) = 1365 * 2^-13; at z = 1: u = fp16(1365 * 2^-13 + 1/2) rounds
+    # 0.66662598 to 1365 * 2^-11, and h = u exactly
+    assert float(np.float16(1.0 / 6.0)) == 1365 * 2.0**-13
+    got = mlp.act_fp16_packed(np.array([1.0, 3.0, -3.0, 0.0, 10.0, -10.0]))
+    assert got[0] == 1365 * 2.0**-11
+    assert got[1] == 3.0                       # u rounds up to 1
+    # at z = -3 the FP16 constant leaves u = 2^-13 (1 - ...) > 0: h != 0,
+    # unlike the exact activation
+    assert got[2] == pytest.approx(-3.0 * 2.0**-13, rel=1e-3) and got[2] < 0
+    assert got[3] == 0.0 and got[4] == 10.0 and got[5] == 0.0
+
+
 def test_fp16_network_between_bf16_and_split():
     """FP16 activations carry 11 bits: their error against the FP32-activation
     network lies between the BF16 (8 bits) and the split BF16 (~16 bits)
