@@ -264,7 +264,6 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
     asm volatile("" : "+v"(zoff));
     const MlpInfer& F =
         *reinterpret_cast<const MlpInfer*>(reinterpret_cast<const char*>(sl.F) + zoff);
-    const mlp::f4 y = mlp::forward<FMT>(F, mlp::input_frag(ain, bin), lane);
     bool ok = fabsf(ain) <= 1.0f && fabsf(bin) <= 1.0f && i > 1 && i < a.Nx - 2 && j > 1 &&
               j < a.Ny - 2;
     if (ok) {
@@ -272,9 +271,15 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
         const int ib = min((int)((bin + 1.0f) * (0.5f * kTrustN)), kTrustN - 1);
         ok = (sl.trust[ib] >> ia) & 1ULL;
     }
-    ok = ok || !real;
-    nfb += (unsigned)__popcll(__ballot(!ok));
+    // the network runs only if some cell of the row uses it (wavefront-
+    // uniform): a row wholly outside the trusted domain takes the exact
+    // routine alone, with the same values
+    const bool any = __ballot(ok && real) != 0;
+    ok = any && (ok || !real);
+    nfb += (unsigned)__popcll(__ballot(real && !ok));
     nall += (unsigned)__popcll(__ballot(real));
+    mlp::f4 y = {0.f, 0.f, 0.f, 0.f};
+    if (any) y = mlp::forward<FMT>(F, mlp::input_frag(ain, bin), lane);
     if (ok) {
         const double onemc = 1.0 - c;
         double lr = 0.0;
