@@ -91,8 +91,33 @@ def cpu_baseline(grid: int, seconds: float):
             "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s)"}
 
 
+def _surrogate_setup(ctx, train_epochs, act_format, weights):
+    """Trains the surrogate, or installs the weights saved at `weights` by an
+    earlier run (and saves them there after training)."""
+    import numpy as np
+    import rsolid
+    if weights and os.path.exists(weights):
+        with np.load(weights, allow_pickle=False) as z:
+            ctx.surrogate_set(z["P"], z["meta"])
+            rep = dict(zip(rsolid.SURR_REPORT_KEYS, (float(v) for v in z["report"])))
+        rep["loaded_from"] = weights
+        return rep
+    kw = {}
+    if train_epochs is not None:
+        kw["epochs"] = train_epochs
+    if act_format is not None:
+        kw["act_format"] = act_format
+    rep = ctx.surrogate_train(**kw)
+    if weights:
+        P, meta = ctx.surrogate_get()
+        np.savez(weights, P=P, meta=meta,
+                 report=np.array([rep[k] for k in rsolid.SURR_REPORT_KEYS], dtype=np.float64))
+    return rep
+
+
 def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
-             train_epochs: int | None = None, act_format: int | None = None):
+             train_epochs: int | None = None, act_format: int | None = None,
+             weights: str | None = None):
     import rsolid
     prec = {"fp64": rsolid.RS_PREC_FP64, "mixed": rsolid.RS_PREC_MIXED,
             "surrogate": rsolid.RS_PREC_SURROGATE}[path]
@@ -101,12 +126,7 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     ctx.init_config(5)
     train = None
     if path == "surrogate":
-        kw = {}
-        if train_epochs is not None:
-            kw["epochs"] = train_epochs
-        if act_format is not None:
-            kw["act_format"] = act_format
-        train = ctx.surrogate_train(**kw)
+        train = _surrogate_setup(ctx, train_epochs, act_format, weights)
     ctx.step(warmup)
     ctx.sync()
     x0 = _tip_x(ctx)
@@ -267,6 +287,9 @@ def main() -> int:
     ap.add_argument("--train-epochs", type=int, default=None,
                     help="surrogate training epochs (default: the library's); profiler runs "
                          "that only need the step kernel's counters use 1")
+    ap.add_argument("--surrogate-weights", default=None,
+                    help="npz file: install the surrogate weights saved there (profiler runs "
+                         "that must not time the training), or train and save them there")
     ap.add_argument("--act-format", default="fp16", choices=sorted(ACT_FORMATS),
                     help="surrogate inference format of the hidden activations")
     ap.add_argument("--ensemble-runs", type=int, default=64,
@@ -288,7 +311,7 @@ def main() -> int:
     for path in order:
         res[path] = run_path(path, a.grid, a.steps, a.warmup,
                              a.calibrate_copy and path == a.path, a.train_epochs,
-                             ACT_FORMATS[a.act_format])
+                             ACT_FORMATS[a.act_format], a.surrogate_weights)
     head = res[a.path]
     dtype = {"fp64": "f64", "mixed": "f32+f64", "surrogate": "bf16/fp16+f32+f64"}[a.path]
     desc = {"fp64": "plain FP64 path",
@@ -339,6 +362,8 @@ def main() -> int:
                             "train_steps": int(r["train"]["steps"]),
                             "trusted_bin_frac": r["train"]["trusted_frac"],
                             "max_residual_trusted": r["train"]["max_res_trusted"]}
+        if "loaded_from" in r["train"]:
+            out["surrogate"]["weights_loaded_from"] = r["train"]["loaded_from"]
     if a.ensemble_runs > 0 and not a.only:
         out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
                                        act_format=ACT_FORMATS[a.act_format])
