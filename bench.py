@@ -67,28 +67,48 @@ def _traffic_from_profile(path: str):
     return None
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo", encoding="utf-8"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(grid: int, seconds: float):
     """The CPU oracle (C, FP64, OpenMP over rows) on the host cores, timed on
-    a bounded sample of the same workload: the first steps of the same grid."""
+    a bounded sample of the same workload: the first steps of the same grid,
+    with up to 16 threads, then a shorter sample on one thread."""
     import oracle
     import rsolid
     cores = min(16, os.cpu_count() or 1)
-    oracle.set_threads(cores)
     p = rsolid.config_params(grid, tfc=True)
-    st = oracle.State(p)
-    st.init_config(5)
-    st.step(1)                     # step 0 (pull-back) outside the timing
-    n, t0 = 0, time.perf_counter()
-    while True:
-        st.step(1)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    st.close()
-    return {"value": round(grid * grid * n / el / 1e6, 3), "unit": "MLUPS", "cores": cores,
-            "kind": "port",
-            "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s)"}
+
+    def sample(threads: int, secs: float):
+        oracle.set_threads(threads)
+        st = oracle.State(p)
+        st.init_config(5)
+        st.step(1)                     # step 0 (pull-back) outside the timing
+        n, t0 = 0, time.perf_counter()
+        while True:
+            st.step(1)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= secs:
+                break
+        st.close()
+        return grid * grid * n / el / 1e6, n, el
+
+    v, n, el = sample(cores, seconds)
+    v1, n1, el1 = sample(1, seconds / 3)
+    oracle.set_threads(cores)
+    return {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s) on "
+                      f"{cores} threads; {n1} steps ({el1:.1f} s) on 1 thread",
+            "value_1thread": round(v1, 3), "cpu_model": _cpu_model(),
+            "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _surrogate_setup(ctx, train_epochs, act_format, weights):

@@ -97,3 +97,23 @@ def test_driver_reference_restart_and_history(rs, tmp_path):
     if nshift:
         h = np.loadtxt(hist, ndmin=2)
         assert h.shape == (nshift, 64)
+
+
+def test_local_launcher_runs_and_logs(rs, tmp_path):
+    """driver/run_local.sh with the reference script's variables: the run
+    directory gets the reference's files and the out.txt log."""
+    # the reference preset (run.sh values) on a 320 x 322 grid, 200 steps
+    p = rs.reference_params(320, 322)
+    k = rs.derive(p)
+    tau0 = (p.S * p.W0) ** 2 / (p.Gamma * p.muk0)
+    T = 200.5 * tau0 * k.dt
+    d = str(tmp_path / "run")
+    script = os.path.join(ROOT, "2d-rapid-solidification_amd", "driver", "run_local.sh")
+    env = dict(os.environ, total_time=repr(T), path_input=d, Nx_all="320", Ny_all="322")
+    say("run_local.sh", d)
+    r = subprocess.run(["bash", script], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    log = open(os.path.join(d, "out.txt"), encoding="utf-8").read()
+    assert log.count("########################################") == 2 and "step" in log
+    assert os.path.exists(os.path.join(d, "tip.txt"))
+    assert os.path.exists(os.path.join(d, "data", "check_point_c.bin"))
