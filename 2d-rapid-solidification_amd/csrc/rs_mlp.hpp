@@ -1,5 +1,6 @@
 // rs_mlp.hpp -- learned CALPHAD surrogate: a 2 -> 64 -> 64 -> 4 MLP trained
-// and evaluated with BF16 MFMA (FP32 accumulate) on gfx950.
+// with BF16 MFMA and evaluated with BF16 or FP16 MFMA (FP32 accumulate) on
+// gfx950.
 //
 // Inputs  (a, b): a = (T - t_mid)/t_half, b = (x_Nb - 0.2)/0.2 with
 //                 x_Nb = 1 - c (c = Ti fraction), i.e. x_Nb in [0, 0.4].
@@ -136,7 +137,7 @@ hipError_t launch_mlp_pack(const float* P, MlpFrags* F, hipStream_t s);
 // one optimiser step over the permuted rows [row0, row0 + nb) of an n-row
 // set (nb a multiple of 64; row r reads sample (r*mult + off) mod n):
 // per-workgroup partial gradients, then a fixed-order reduce + Adam on the
-// FP32 master weights, then the BF16 fragments are repacked
+// FP32 master weights, then the weight fragments are repacked
 hipError_t launch_mlp_train_step(const float* X, const float* Yn, int64_t n, int64_t row0, int nb,
                                  int64_t mult, int64_t off, MlpFrags* F, float* partial,
                                  float* loss_part, float* P, float* m, float* v, float lr, float b1,

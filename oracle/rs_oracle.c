@@ -614,20 +614,23 @@ static void or_compute_c(or_state* s) {
  * and c_next -= dt (div J), then the clamp at 1e-6 and the ghost copies. */
 double or_at_flux(const rs_params* p, double vdn, double dt, double pP, double pQ, double nP,
                   double nQ, double cP, double cQ, double t1, double t2, double t3, double t4) {
-    const double dx = p->dx;
-    const double gn = (pQ - pP) / dx;
-    const double gt = ((t1 + t2) - (t3 + t4)) / (4.0 * dx);
+    /* J = a_t (1-k) c_L dphidt (-g_n/|g|) / (1 + |V|/V_D), c_L = c_f/(1 - h_f (1-k)),
+     * written with the reciprocals of dx, 4 dx, dt, V_D and of |g|, and c_L
+     * and the trapping factor under one division */
+    const double idx = 1.0 / p->dx, i4dx = 1.0 / (4.0 * p->dx), idt = 1.0 / dt;
+    const double ivdn = vdn > 0.0 ? 1.0 / vdn : 0.0;
+    const double omk = 1.0 - p->k_part, coef = p->at_coef * (1.0 - p->k_part);
+    const double gn = (pQ - pP) * idx;
+    const double gt = ((t1 + t2) - (t3 + t4)) * i4dx;
     const double g2 = gn * gn + gt * gt;
     if (!(g2 > 1e-20)) return 0.0;
-    const double g = sqrt(g2);
-    const double dpt = 0.5 * ((nP - pP) + (nQ - pQ)) / dt;
-    double trap = 1.0;
-    if (vdn > 0.0) trap = 1.0 / (1.0 + fabs(dpt / g) / vdn);
-    const double k = p->k_part;
+    const double ig = 1.0 / sqrt(g2);
+    const double dpt = 0.5 * ((nP - pP) + (nQ - pQ)) * idt;
     const double cf = 0.5 * (cP + cQ);
     const double hf = 0.5 * (1.0 + 0.5 * (pP + pQ));
-    const double cl = cf / (1.0 - hf * (1.0 - k));
-    return p->at_coef * (1.0 - k) * cl * dpt * (-gn / g) * trap;
+    double den = 1.0 - hf * omk;
+    if (ivdn > 0.0) den = den * (1.0 + fabs(dpt * ig) * ivdn);
+    return coef * (cf / den) * dpt * (-gn * ig);
 }
 
 static void or_antitrap(or_state* s) {
@@ -658,7 +661,7 @@ static void or_antitrap(or_state* s) {
             const double Jyp = or_at_flux(p, vdn, dt, F(i, j), F(i, j + 1), G(i, j), G(i, j + 1),
                                           CC(i, j), CC(i, j + 1), F(i + 1, j), F(i + 1, j + 1),
                                           F(i - 1, j), F(i - 1, j + 1));
-            const double dc = -dt * ((Jxp - Jxm) + (Jyp - Jym)) / p->dx;
+            const double dc = -dt * ((Jxp - Jxm) + (Jyp - Jym)) * (1.0 / p->dx);
             s->c_next[POS(s, i, j)] = fmax(s->c_next[POS(s, i, j)] + dc, 1.0e-6);
         }
     }
