@@ -369,7 +369,9 @@ int rs_create(const rs_params* p, rs_ctx** out) {
              hipMalloc(&ctx->psi32[1], fb32) == hipSuccess &&
              hipMalloc(&ctx->T32[0], tb32) == hipSuccess && hipMalloc(&ctx->T32[1], tb32) == hipSuccess &&
              hipMemsetAsync(ctx->psi32[0], 0, fb32, ctx->stream) == hipSuccess &&
-             hipMemsetAsync(ctx->psi32[1], 0, fb32, ctx->stream) == hipSuccess;
+             hipMemsetAsync(ctx->psi32[1], 0, fb32, ctx->stream) == hipSuccess &&
+             hipMemsetAsync(ctx->T32[0], 0, tb32, ctx->stream) == hipSuccess &&
+             hipMemsetAsync(ctx->T32[1], 0, tb32, ctx->stream) == hipSuccess;
     } else {
         ok = hipMalloc(&ctx->psi[0], fb) == hipSuccess && hipMalloc(&ctx->psi[1], fb) == hipSuccess &&
              hipMemsetAsync(ctx->psi[0], 0, fb, ctx->stream) == hipSuccess &&
@@ -754,7 +756,8 @@ int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms) {
     double tot = 0.0;
     for (int q = 0; q < reps; ++q) {
         HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
-        HIPCHK(ctx, launch_copy(ctx->c[0], ctx->c[1], n, ctx->stream));
+        // current c into the other buffer, which the next step overwrites
+        HIPCHK(ctx, launch_copy(ctx->c[ctx->cur], ctx->c[ctx->cur ^ 1], n, ctx->stream));
         HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
         HIPCHK(ctx, hipEventSynchronize(e1));
         float ms = 0.f;

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <numeric>
 #include <vector>
 
 #include "rs_host.hpp"
@@ -423,7 +424,7 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
         int64_t mult;
         do {
             mult = (int64_t)(splitmix(&st) % (uint64_t)n) | 1;
-        } while (std::__gcd(mult, n) != 1);
+        } while (std::gcd(mult, n) != 1);
         const int64_t off = (int64_t)(splitmix(&st) % (uint64_t)n);
         for (int64_t b = 0; b < spe; ++b) {
             ++t;

@@ -251,9 +251,10 @@ int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
 
 /* Measurement helper with no counterpart in the reference: launches a plain
  * FP64 copy kernel (n doubles read, n written, n <= Nx*Ny) reps times on
- * the context stream between the context's two psi buffers, so profiler
- * byte counters can be calibrated against a known traffic. Destroys the
- * field contents; writes the average kernel time (HIP events). */
+ * the context stream, from the current c buffer into the other one (which
+ * the next step overwrites; the state is kept), so profiler byte counters
+ * can be calibrated against a known traffic. Writes the average kernel
+ * time (HIP events). */
 int rs_calibrate_copy(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms);
 
 /* ---- on-disk formats of the reference's driver (host only) --------------
