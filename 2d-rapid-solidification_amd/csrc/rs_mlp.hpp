@@ -35,11 +35,11 @@
 // same (lane group, element) -> k map for A and B. Hence:
 //  * layer 1 (K32): each input is split into three BF16 parts, (a_hi,
 //    a_mid, a_lo, b_hi, b_mid, b_lo, 0, 0), which carry the full FP32 value;
-//    the inputs of tile t are moved into lane group 0 (ds_bpermute) and the
-//    other groups are zeroed, so every tile uses the same k slots and a
-//    cell's result does not depend on its lane (two wavefronts that both
-//    hold a halo cell must compute the same value for the solute fluxes to
-//    stay antisymmetric);
+//    the inputs of tile t are moved into lane group 0 (ds_bpermute; the
+//    other groups get copies that meet zero weights), so every tile uses
+//    the same k slots and a cell's result does not depend on its lane (two
+//    wavefronts that both hold a halo cell must compute the same value for
+//    the solute fluxes to stay antisymmetric);
 //  * layer 2 (K32): two D tiles of layer 1 concatenated are a B fragment; the
 //    32 hidden units of K block kb sit at (group g, element e) ->
 //    sigma(kb, g, e) = 32 kb + 16 (e/4) + 4 g + e%4, and W2 is packed with the
@@ -251,7 +251,9 @@ __device__ __forceinline__ uint4 input_frag(float a, float b) {
                       (m >> 16) | (l & 0xFFFF0000u), 0u);
 }
 
-// the input fragments of tile t in lane group 0, zero elsewhere
+// the input fragments of tile t in every lane group; the layer-1 weight
+// fragment is zero outside the k slots of group 0, so the copies in groups
+// 1..3 (finite BF16 values) contribute exact zeros
 __device__ __forceinline__ uint4 tile_inputs(uint4 xin, int t, int lane) {
     const int src = (16 * t + (lane & 15)) << 2;
     uint4 r;
@@ -259,7 +261,6 @@ __device__ __forceinline__ uint4 tile_inputs(uint4 xin, int t, int lane) {
     r.y = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)xin.y);
     r.z = (uint32_t)__builtin_amdgcn_ds_bpermute(src, (int)xin.z);
     r.w = 0u;
-    if (lane >= 16) r = make_uint4(0u, 0u, 0u, 0u);
     return r;
 }
 // layer 1 of tile t: z1[mt] = W1' x + b1 for the 16 samples of tile t
