@@ -248,7 +248,7 @@ int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
     rs_derive(&p, &ctx->k);
     ctx->cur = 0;
     ctx->tcur = 0;
-    orientation(ctx->orient, 0.0, 0.0, 0.0);
+    orientation(ctx->orient, ctx->p.alpha0, ctx->p.beta0, ctx->p.gamma0);
     HIPCHK(ctx, hipMemcpyAsync(ctx->x0, x0.data(), sizeof(double) * p.Ny, hipMemcpyHostToDevice,
                                ctx->stream));
     if (ctx->mixed)
@@ -336,7 +336,7 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     ctx->p = *p;
     rs_derive(p, &ctx->k);
     ctx->d = derive_constants(*p, ctx->k.dt);
-    orientation(ctx->orient, 0.0, 0.0, 0.0);
+    orientation(ctx->orient, ctx->p.alpha0, ctx->p.beta0, ctx->p.gamma0);
     ctx->n = (int64_t)p->Nx * p->Ny;
     ctx->nT = (int64_t)ctx->k.NxT_all * ctx->k.NyT_all;
     if (const char* e = getenv("RS_BAND_H")) {

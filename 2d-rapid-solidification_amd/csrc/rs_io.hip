@@ -114,7 +114,7 @@ int rs_read_checkpoint(const char* path, double* data, int64_t n) {
 // parameters.txt (save_parameters, fun.cu:2060-2202): the same lines in the
 // same order, so the reference's loader reads it; quantities this build has
 // no parameter for (dilute-alloy d0/ke/b/lambda, molar weight, density,
-// orientation angles, eps2, random seed) are written as 0.
+// the second grain's angle, eps2, random seed) are written as 0.
 int rs_write_parameters(const char* path, const rs_params* p, const rs_constants* k, int32_t i_tip) {
     if (!path || !p || !k) return RS_ERR_ARG;
     File out(path, "w");
@@ -162,7 +162,7 @@ int rs_write_parameters(const char* path, const rs_params* p, const rs_constants
     fprintf(fp, "%-13g b=ln(ke)/2, only for dilute alloys\n", 0.0);
     fprintf(fp, "%-13g lambda, only for dilute alloys\n", 0.0);
     fprintf(fp, "\n");
-    fprintf(fp, "%-13g gamma0 [degrees], orientation angle\n", 0.0);
+    fprintf(fp, "%-13g gamma0 [degrees], orientation angle\n", p->gamma0);
     fprintf(fp, "%-13g gamma1 [degrees], orientation angle\n", 0.0);
     fprintf(fp, "%-13g eps1, anisotropy parameter of the interfacial free energy \n", p->eps1);
     fprintf(fp, "%-13g eps2, anisotropy parameter of the interfacial free energy \n", 0.0);

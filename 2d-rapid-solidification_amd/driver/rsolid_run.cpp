@@ -8,7 +8,8 @@
 // time: Nx_all Ny_all total_time Vp GG eps1 epk1 path_input if_load
 // run_time. Further keys: preset=reference|config, N, tfc, alpha (config
 // preset), precision=fp64|mixed|surrogate, seed (config initialiser), at_coef V_D
-// k_part (antitrapping/trapping extension), di dj (output stride),
+// k_part (antitrapping/trapping extension), alpha0 beta0 gamma0 (grain
+// orientation, degrees), di dj (output stride),
 // state=1 (also write the exact state file for a bit-exact restart),
 // load_state=FILE (restart from such a file instead of the checkpoints),
 // pull_back=N (pull-back cadence in steps instead of the derived one).
@@ -176,6 +177,9 @@ int main(int argc, char** argv) {
     r.p.at_coef = argd("at_coef", r.p.at_coef);
     r.p.V_D = argd("V_D", r.p.V_D);
     r.p.k_part = argd("k_part", r.p.k_part);
+    r.p.alpha0 = argd("alpha0", r.p.alpha0);  // fun.cu:129-131 grain orientation
+    r.p.beta0 = argd("beta0", r.p.beta0);
+    r.p.gamma0 = argd("gamma0", r.p.gamma0);
     const std::string prec = arg("precision", "fp64");
     r.p.precision = prec == "mixed"       ? RS_PREC_MIXED
                     : prec == "surrogate" ? RS_PREC_SURROGATE

@@ -46,7 +46,8 @@ class RsParams(C.Structure):
         ("L1_liq", _D), ("dt_coef", _D), ("dtT_coef", _D), ("length_sample_um", _D),
         ("lap_w_axis", _D), ("lap_w_diag", _D), ("q_solid", _D), ("total_time", _D),
         ("delta_initial", _D), ("phi_interface", _D), ("c_liq0", _D), ("c_sol0", _D),
-        ("at_coef", _D), ("V_D", _D), ("k_part", _D),
+        ("at_coef", _D), ("V_D", _D), ("k_part", _D), ("alpha0", _D), ("beta0", _D),
+        ("gamma0", _D),
     ]
 
     def as_dict(self) -> dict:

@@ -127,6 +127,10 @@ typedef struct rs_params {
     double at_coef;            /* a_t; 1/(2 sqrt 2) for phi in [-1, 1]       */
     double V_D;                /* diffusive speed, m/s; 0 = no trapping term */
     double k_part;             /* equilibrium partition c_S / c_L of c       */
+    /* Euler angles of the grain, degrees (fun.cu:129-131; 0 in the reference,
+     * applied by set_orientation at initialisation, fun.cu:1017); any non-zero
+     * angle selects the general-orientation anisotropy kernels */
+    double alpha0, beta0, gamma0;
 } rs_params;
 
 /* Run state: field-for-field the reference's struct constants

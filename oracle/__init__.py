@@ -41,6 +41,7 @@ def _load() -> C.CDLL:
         "or_Gl": (d, [P(RsParams), d, d]), "or_Gs": (d, [P(RsParams), d, d]),
         "or_Glc": (d, [P(RsParams), d, d]), "or_Gsc": (d, [P(RsParams), d, d]),
         "or_anisotropy": (d, [P(RsParams), P(d), P(d), P(d), P(d)]),
+        "or_orientation": (None, [P(d), d, d, d]),
         "or_create": (vp, [P(RsParams)]),
         "or_destroy": (None, [vp]),
         "or_init_reference": (C.c_int, [vp]),
@@ -121,6 +122,13 @@ def anisotropy(p: RsParams, ph3x3: np.ndarray, orient=None):
     a_k = C.c_double()
     v = lib.or_anisotropy(_P(p), _dptr(ph), _dptr(o), C.byref(a_s), C.byref(a_k))
     return v, a_s.value, a_k.value
+
+
+def orientation(alpha: float, beta: float, gamma: float) -> np.ndarray:
+    """set_orientation (fun.cu:750-771): 3x3 rotation from Euler angles (deg)."""
+    o = np.zeros(9)
+    lib.or_orientation(_dptr(o), alpha, beta, gamma)
+    return o.reshape(3, 3)
 
 
 def at_flux(p, vdn, dt, pP, pQ, nP, nQ, cP, cQ, t1=0.0, t2=0.0, t3=0.0, t4=0.0) -> float:

@@ -319,7 +319,7 @@ void or_destroy(or_state* s) {
 #define PST(s, i, j) ((int64_t)(j) * (s)->k.NxT_all + (i))
 
 /* set_orientation (fun.cu:750-771) */
-static void or_orientation(double* o, double al, double be, double ga) {
+void or_orientation(double* o, double al, double be, double ga) {
     al = al * PI_REF / 180.0;
     be = be * PI_REF / 180.0;
     ga = ga * PI_REF / 180.0;
@@ -391,7 +391,7 @@ int or_init_reference(or_state* s) {
             s->psi[POS(s, i, j)] = 0.0 + ph;
         }
     }
-    or_orientation(s->orient, 0, 0, 0);
+    or_orientation(s->orient, s->p.alpha0, s->p.beta0, s->p.gamma0);
     if (s->p.ifTFC) {
         or_init_T(s);
         or_interpolate(s, s->k.i_left_end);
@@ -432,7 +432,7 @@ int or_init_config(or_state* s, uint64_t seed) {
         }
     }
     free(x0);
-    or_orientation(s->orient, 0, 0, 0);
+    or_orientation(s->orient, s->p.alpha0, s->p.beta0, s->p.gamma0);
     if (s->p.ifTFC) {
         or_init_T(s);
         or_interpolate(s, s->k.i_left_end);

@@ -42,6 +42,10 @@ double or_Gs(const rs_params* p, double T, double c);
 double or_Glc(const rs_params* p, double T, double c);
 double or_Gsc(const rs_params* p, double T, double c);
 
+/* set_orientation (fun.cu:750-771): the 9-entry rotation of Euler angles
+ * (degrees) with the float-precision PI of fun.cu:59. */
+void or_orientation(double* o, double alpha, double beta, double gamma);
+
 /* get_anisotropy (fun.cu:1030-1139) on a 3x3 phi neighbourhood given as
  * ph[3*dj + di] with di,dj in {0,1,2} (centre = ph[4]); orientation is the
  * 9-entry rotation (fun.cu:750-771). Returns the anisotropic term, writes

@@ -28,11 +28,14 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
 
 
-@pytest.mark.parametrize("tfc", [False, True])
-def test_mixed_tracks_fp64_oracle(rs, oracle_mod, tfc):
+@pytest.mark.parametrize("tfc,rotated", [(False, False), (True, False), (True, True)])
+def test_mixed_tracks_fp64_oracle(rs, oracle_mod, tfc, rotated):
     pm = rs.config_params(256, tfc, precision=RS_PREC_MIXED)
     pd = rs.config_params(256, tfc)
-    say("mixed vs oracle, 256^2, tfc", tfc)
+    if rotated:  # a grain turned away from the axes: the general-orientation kernels
+        for p in (pm, pd):
+            p.alpha0, p.beta0, p.gamma0 = 10.0, 15.0, 25.0
+    say("mixed vs oracle, 256^2, tfc", tfc, "rotated", rotated)
     cm = rs.Context(pm)
     cm.init_config(7)
     info = cm.info()

@@ -28,7 +28,14 @@ def _cases(rs):
         "ref512x272": (rs.reference_params(512, 272), "ref", 0),
         "cfg256_frozen": (rs.config_params(256, False), "cfg", 7),
         "cfg256_coupled": (rs.config_params(256, True), "cfg", 7),
+        "cfg256_rotated": (_rotated(rs.config_params(256, True)), "cfg", 7),
     }
+
+
+def _rotated(p):
+    # a grain turned away from the axes (fun.cu:129-131 Euler angles)
+    p.alpha0, p.beta0, p.gamma0 = 10.0, 15.0, 25.0
+    return p
 
 
 def _init(obj, kind, seed):
@@ -98,11 +105,11 @@ def test_single_step_from_identical_state(rs, oracle_mod, case):
 
 
 @pytest.mark.parametrize("case,steps", [("cfg256_frozen", 2000), ("cfg256_coupled", 2000),
-                                        ("ref512x272", 2100)])
+                                        ("ref512x272", 2100), ("cfg256_rotated", 2000)])
 def test_run_matches_oracle(rs, oracle_mod, case, steps):
     """configs[0] (256^2 frozen T, 2000 steps), the same grid with the coupled
-    heat equation, and the reference parameter set with its step-0, 1025 and
-    2050 pull-backs."""
+    heat equation (also with a rotated grain), and the reference parameter
+    set with its step-0, 1025 and 2050 pull-backs."""
     p, kind, seed = _cases(rs)[case]
     say("run", case, steps, "steps")
     st = oracle_mod.State(p)

@@ -127,6 +127,46 @@ def test_anisotropy_strength_closed_form(rs, oracle_mod, theta):
     assert abs(v) < 1e-12
 
 
+PI_REF = 3.1415927  # fun.cu:59
+
+
+def test_orientation_is_a_rotation(oracle_mod):
+    assert np.array_equal(oracle_mod.orientation(0.0, 0.0, 0.0), np.eye(3))
+    for ang in ((10.0, 15.0, 25.0), (45.0, 0.0, 30.0), (3.0, 44.0, 7.0)):
+        R = oracle_mod.orientation(*ang)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-15)
+        assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-15)
+    # alpha = beta = 0: a rotation by gamma about z (fun.cu:750-771)
+    g = 25.0 * PI_REF / 180.0
+    Rz = np.array([[math.cos(g), -math.sin(g), 0.0], [math.sin(g), math.cos(g), 0.0],
+                   [0.0, 0.0, 1.0]])
+    assert np.allclose(oracle_mod.orientation(0.0, 0.0, 25.0), Rz, atol=1e-16)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 1.0, 2.5])
+def test_anisotropy_of_a_rotated_grain(rs, oracle_mod, theta):
+    """Linear phi along theta: the rotated normal n' = R (cos, sin, 0) enters
+    a_s = 1 - 3 eps + 4 eps sum n'_k^4. About z (gamma): the 4-fold pattern
+    turns, a_s = 1 + eps cos 4(theta + gamma); about y (beta): n' = (cos b
+    cos t, sin t, -sin b cos t)."""
+    eps, epk = 0.05, 0.13
+    p = _params_eps(rs, eps, epk)
+    x = np.arange(3) - 1.0
+    X, Y = np.meshgrid(x, x)
+    ph = 0.1 * (math.cos(theta) * X + math.sin(theta) * Y) * p.dx
+    g = 25.0 * PI_REF / 180.0
+    v, a_s, a_k = oracle_mod.anisotropy(p, ph, oracle_mod.orientation(0.0, 0.0, 25.0))
+    assert a_s == pytest.approx(1 + eps * math.cos(4 * (theta + g)), abs=1e-13)
+    assert a_k == pytest.approx(1 + epk * math.cos(4 * (theta + g)), abs=1e-13)
+    assert abs(v) < 1e-12
+    b = 15.0 * PI_REF / 180.0
+    n = (math.cos(b) * math.cos(theta), math.sin(theta), -math.sin(b) * math.cos(theta))
+    np4 = sum(q ** 4 for q in n)
+    _, a_s, a_k = oracle_mod.anisotropy(p, ph, oracle_mod.orientation(0.0, 15.0, 0.0))
+    assert a_s == pytest.approx(1 - 3 * eps + 4 * eps * np4, abs=1e-13)
+    assert a_k == pytest.approx(1 - 3 * epk + 4 * epk * np4, abs=1e-13)
+
+
 def test_anisotropy_isotropic_limits(rs, oracle_mod):
     rng = np.random.default_rng(1)
     ph = rng.uniform(-1, 1, (3, 3))
