@@ -26,7 +26,7 @@ struct File {
     }
 };
 
-constexpr uint64_t kStateMagic = 0x3153544153525352ULL;  // "RSRSATS1"
+constexpr uint64_t kStateMagic = 0x3253544153525352ULL;  // "RSRSATS2" (rs_params with grain angles)
 
 }  // namespace
 
