@@ -17,6 +17,7 @@ RS_ERR_STATE = -5
 RS_PREC_FP64 = 0
 RS_PREC_MIXED = 1
 RS_PREC_SURROGATE = 2
+RS_ABI_VERSION = 2  # include/rsolid.h: the layouts below
 
 RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED = 0, 1, 2, 3
 RS_SURR_NPARAM = 4612

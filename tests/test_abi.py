@@ -7,7 +7,7 @@ import re
 
 import pytest
 
-from rsolid.abi import EXPORTED, RsConstants, RsParams, RsTipRow
+from rsolid.abi import EXPORTED, RS_ABI_VERSION, RsConstants, RsParams, RsTipRow
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -31,7 +31,7 @@ def test_struct_layouts(rs):
     assert s[0] == C.sizeof(RsParams)
     assert s[1] == C.sizeof(RsConstants)
     assert s[2] == C.sizeof(RsTipRow)
-    assert s[3] == 1
+    assert s[3] == RS_ABI_VERSION
 
 
 @pytest.mark.parametrize("maker", ["ref768", "ref512", "cfg256", "cfg4096t", "cfg2048t"])
