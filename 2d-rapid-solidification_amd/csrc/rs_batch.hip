@@ -70,7 +70,7 @@ int build_tables(rs_batch* b) {
     b->h_runs.resize(n);
     b->h_dyn.resize(n);
     std::vector<int> item0(n + 1, 0), cell0(n + 1, 0);
-    bool ident = true;
+    bool ident = true, at_any = false;
     for (int r = 0; r < n; ++r) {
         rs_ctx* c = b->runs[r];
         c->band_h = b->band_h[r];
@@ -107,6 +107,10 @@ int build_tables(rs_batch* b) {
         R.GG = p.GG;
         R.l_off0 = k.l_offset_Vp_initial;
         R.gstep = p.GG * p.dx * p.S * p.W0;
+        const double vdn = p.V_D > 0.0 ? p.V_D * R.tau0 / (p.S * p.W0) : 0.0;
+        R.at = at_args(p.dx, k.dt, p.at_coef, p.k_part, vdn);
+        R.at_on = p.at_coef != 0.0 ? 1 : 0;
+        at_any = at_any || R.at_on;
         BatchDyn& d = b->h_dyn[r];
         d.step = k.step;
         d.l_offset_Vp = k.l_offset_Vp;
@@ -138,6 +142,7 @@ int build_tables(rs_batch* b) {
     L.items = item0[n];
     L.cells = cell0[n];
     L.ident = ident;
+    L.at_any = at_any ? 1 : 0;
     const rs_ctx* c0 = b->runs[0];
     L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
     if (L.surr) L.surr = 1 + b->h_runs[0].a.s_fmt;
@@ -323,7 +328,6 @@ int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out) {
         if (params[r].precision != RS_PREC_MIXED && params[r].precision != RS_PREC_SURROGATE)
             return RS_ERR_ARG;  // the batched kernels are the FP32-field kernels
         if (params[r].precision != params[0].precision) return RS_ERR_ARG;
-        if (params[r].at_coef != 0.0) return RS_ERR_ARG;  // no antitrapping pass in the batch
     }
     rs_batch* b = new (std::nothrow) rs_batch;
     if (!b) return RS_ERR_NOMEM;

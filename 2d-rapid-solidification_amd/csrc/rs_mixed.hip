@@ -733,6 +733,25 @@ __global__ void batch_advance_kernel(BatchLaunch L) {
     L.dyn[r] = d;
 }
 
+// the antitrapping pass of the runs that have it, over the step's work list
+// (same bands; the band split does not change the result)
+__global__ __launch_bounds__(256) void batch_antitrap_kernel(BatchLaunch L) {
+    for (int item = blockIdx.x; item < L.items; item += gridDim.x) {
+        const int r = batch_run_of(L.item0, L.nruns, item);
+        const BatchRun& R = L.runs[r];
+        if (!R.at_on) continue;
+        const BatchDyn d = L.dyn[r];
+        const int local = item - L.item0[r];
+        const int tx = local % R.ntx;
+        const int band = (local / R.ntx) * blockDim.y + threadIdx.y;
+        if (band >= R.a.nbands) continue;
+        const int jb = 1 + band * R.a.band_h;
+        const int je = min(jb + R.a.band_h - 1, R.a.Ny - 2);
+        antitrap_band<float>(R.psi[d.cur], R.psi[d.cur ^ 1], R.c[d.cur], R.c[d.cur ^ 1], R.a.Nx,
+                             R.a.Ny, R.at, tx, jb, je);
+    }
+}
+
 template <bool IDENT, int SURR>
 static hipError_t batch_step_t(const BatchLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL((batch_step_kernel<IDENT, SURR>), dim3(L.step_blocks), dim3(kWave, L.wpb),
@@ -755,6 +774,11 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
     default: e = batch_step_t<true, 4>(L, s); break;
     }
     if (e != hipSuccess) return e;
+    if (L.at_any) {
+        hipLaunchKernelGGL(batch_antitrap_kernel, dim3(L.step_blocks), dim3(kWave, L.wpb), 0, s, L);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     if (L.cells > 0) {
         hipLaunchKernelGGL(batch_temperature_kernel, dim3(L.T_blocks), dim3(256), 0, s, L);
         e = hipGetLastError();

@@ -3,6 +3,7 @@
 // chemical potential and all time accumulation in FP64.
 #pragma once
 
+#include "rs_antitrap.hpp"
 #include "rs_kernels.hpp"
 #include "rs_mlp.hpp"
 
@@ -76,6 +77,8 @@ struct BatchRun {
     double Vp, Vp_rate, tau0, dt, S, W0, SW0;
     int ramp, ioff;
     double Tmelt, deltaT0, delta_initial, GG, l_off0, gstep;
+    AtArgs at;        // antitrapping correction (extension), if at_on
+    int at_on;
 };
 // Per-run state advanced on the device by the captured step sequence
 struct BatchDyn {
@@ -92,8 +95,10 @@ struct BatchLaunch {
     int step_blocks, wpb, T_blocks;
     bool ident;
     int surr;           // 0 exact, else 1 + the surrogate's activation format
+    int at_any;         // some run has the antitrapping correction on
 };
-// one step of every run: fused step kernel, coarse T kernel, state advance
+// one step of every run: fused step kernel, antitrapping pass (runs with
+// it on), coarse T kernel, state advance
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);
 // blocks of the step kernel resident per CU for waves-per-block wpb
 int batch_step_resident(const BatchLaunch& L);

@@ -383,9 +383,9 @@ int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset);
  * -- any grid sizes, undercoolings and pulling speeds, one precision path
  * (RS_PREC_MIXED or RS_PREC_SURROGATE) -- advance together: each step is one
  * persistent launch of the fused step kernel over the work list of all runs
- * (run, column tile, band group), one coarse-T launch and a state-advance
- * launch, and the step sequence is captured once in a HIP graph and
- * replayed. Pull-backs (fun.cu:448) split the replay into segments and run
+ * (run, column tile, band group), the antitrapping pass of the runs that
+ * have it on, one coarse-T launch and a state-advance launch, and the step
+ * sequence is captured once in a HIP graph and replayed. Pull-backs (fun.cu:448) split the replay into segments and run
  * on the host between them. Results are bitwise those of the runs stepped
  * one by one with rs_step. Coupled-T runs need ratio_dt_dtT == 1. */
 typedef struct rs_batch rs_batch;

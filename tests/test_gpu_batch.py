@@ -68,6 +68,26 @@ def test_batch_matches_single_runs(rs, pull_back):
     b.close()
 
 
+def test_batch_antitrapping_matches_single_runs(rs):
+    """Runs with the antitrapping / solute-trapping correction (two of three,
+    different V_D) and one without, batched against rs_step, pull-backs on."""
+    params = [rs.ensemble_params(n, dT, v, RS_PREC_MIXED) for n, dT, v in SPECS[:3]]
+    for p, vd in zip(params[:2], (0.5, 0.0)):
+        p.at_coef = 1.0 / (2.0 * np.sqrt(2.0))
+        p.V_D = vd
+    seeds = [31, 32, 33]
+    b = rs.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+        k = run.constants()
+        k.step_pull_back = 120
+        run.set_constants(k)
+    say("antitrapping batch")
+    b.step(250)
+    _check(b, _singles(rs, params, seeds, 250, 120))
+    b.close()
+
+
 def test_batch_tune_keeps_state_and_results(rs):
     params = [rs.ensemble_params(n, dT, v, RS_PREC_MIXED) for n, dT, v in SPECS]
     seeds = [21, 22, 23, 24]
