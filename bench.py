@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -133,6 +134,37 @@ def _surrogate_setup(ctx, train_epochs, act_format, weights):
         np.savez(weights, P=P, meta=meta,
                  report=np.array([rep[k] for k in rsolid.SURR_REPORT_KEYS], dtype=np.float64))
     return rep
+
+
+def run_antitrapping(grid: int, steps: int, warmup: int) -> dict:
+    """The antitrapping / solute-trapping extension (a_t = 1/(2 sqrt 2),
+    V_D = 2 Vp) on the FP64 and FP32 paths of the same workload: step
+    throughput and the correction pass's HIP-event time (step kernel + pass,
+    minus the plain step kernel)."""
+    import rsolid
+    out = {}
+    for name, prec in (("fp64", rsolid.RS_PREC_FP64), ("mixed", rsolid.RS_PREC_MIXED)):
+        res = {}
+        for at in (0.0, 1.0 / (2.0 * math.sqrt(2.0))):
+            p = rsolid.config_params(grid, tfc=True, precision=prec)
+            p.at_coef = at
+            p.V_D = 2.0 * p.Vp if at else 0.0
+            ctx = rsolid.Context(p)
+            ctx.init_config(5)
+            ctx.step(warmup)
+            ctx.sync()
+            ctx.timing_enable(True, every=TIMING_EVERY)
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.sync()
+            el = time.perf_counter() - t0
+            k_ms, _ = ctx.timing(0)
+            ctx.close()
+            res[at] = (grid * grid * steps / el / 1e6, k_ms)
+        (v0, k0), (v1, k1) = res[0.0], res[max(res)]
+        out[name] = {"value": round(v1, 2), "unit": "MLUPS", "without": round(v0, 2),
+                     "pass_ms": round(k1 - k0, 5)}
+    return out
 
 
 def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
@@ -384,6 +416,8 @@ def main() -> int:
                             "max_residual_trusted": r["train"]["max_res_trusted"]}
         if "loaded_from" in r["train"]:
             out["surrogate"]["weights_loaded_from"] = r["train"]["loaded_from"]
+    if not a.only:
+        out["antitrapping"] = run_antitrapping(a.grid, 300, 20)
     if a.ensemble_runs > 0 and not a.only:
         out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
                                        act_format=ACT_FORMATS[a.act_format])
