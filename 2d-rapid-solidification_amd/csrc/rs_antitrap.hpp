@@ -81,22 +81,25 @@ __device__ __forceinline__ double at_dn(double v) {
 }
 
 // one row of the window: phi (= max(psi, -1)) with its i-1 / i+1
-// neighbours, phi_next and c with their i+1 neighbours
+// neighbours, phi_next and c with their i+1 neighbours, and the row's
+// uncorrected c_next
 template <typename R>
 struct AtRow {
     R f, fl, fr, g, gr, c, cr;
+    double cn;
 };
-// the raw loads of one row, issued a row ahead of their use
+// the raw loads of one row, issued two rows ahead of the row's update (c_next
+// included: the update then waits on no load)
 template <typename T>
 struct AtRaw {
     T f, g;
-    double c;
+    double c, cn;
 };
 template <typename T>
-__device__ __forceinline__ AtRaw<T> at_load(const T* psi, const T* psi_n, const double* c, int Nx,
-                                            int j, int ic) {
+__device__ __forceinline__ AtRaw<T> at_load(const T* psi, const T* psi_n, const double* c,
+                                            const double* c_n, int Nx, int j, int ic) {
     const int64_t q = (int64_t)j * Nx + ic;
-    return AtRaw<T>{psi[q], psi_n[q], c[q]};
+    return AtRaw<T>{psi[q], psi_n[q], c[q], c_n[q]};
 }
 template <typename R, typename T>
 __device__ __forceinline__ AtRow<R> at_row(const AtRaw<T>& w) {
@@ -104,6 +107,7 @@ __device__ __forceinline__ AtRow<R> at_row(const AtRaw<T>& w) {
     r.f = fmax((R)w.f, R(-1));
     r.g = fmax((R)w.g, R(-1));
     r.c = (R)w.c;
+    r.cn = w.cn;
     r.fl = at_up(r.f);
     r.fr = at_dn(r.f);
     r.gr = at_dn(r.g);
@@ -126,7 +130,10 @@ __device__ __forceinline__ R at_yface(const AtConst<R>& a, const AtRow<R>& Z, co
 // recomputed from the same operands, so any band split gives the same
 // bits. The flux arithmetic follows the field precision T: FP64 with FP64
 // psi, FP32 with FP32 psi. c_n holds the step's new c and is corrected in
-// place (clamp at 1e-6, ghost images by the owning cell).
+// place (clamp at 1e-6, ghost images by the owning cell); an interior cell's
+// c_n is read and written only by the lane that owns it, so the window's
+// early loads of the rows below read uncorrected values (halo lanes load
+// values they never use).
 template <typename T>
 __device__ __forceinline__ void antitrap_band(const T* psi, const T* psi_n, const double* c,
                                               double* c_n, int Nx, int Ny, const AtArgs& a,
@@ -138,20 +145,20 @@ __device__ __forceinline__ void antitrap_band(const T* psi, const T* psi_n, cons
     const int i = tile * kTileOut + lane;
     const int ic = i < Nx ? i : Nx - 1;
     const bool own = lane >= 1 && lane <= kTileOut && i <= Nx - 2;
-    AtRow<R> M = at_row<R>(at_load(psi, psi_n, c, Nx, jb - 1, ic));
-    AtRow<R> Z = at_row<R>(at_load(psi, psi_n, c, Nx, jb, ic));
-    AtRaw<T> nx = at_load(psi, psi_n, c, Nx, jb + 1, ic);
+    AtRow<R> M = at_row<R>(at_load(psi, psi_n, c, c_n, Nx, jb - 1, ic));
+    AtRow<R> Z = at_row<R>(at_load(psi, psi_n, c, c_n, Nx, jb, ic));
+    AtRaw<T> nx = at_load(psi, psi_n, c, c_n, Nx, jb + 1, ic);
     double Jy_prev = (double)at_yface(k, M, Z);
     for (int j = jb; j <= je; ++j) {
         const AtRow<R> P = at_row<R>(nx);
-        if (j + 2 <= je + 1) nx = at_load(psi, psi_n, c, Nx, j + 2, ic);
+        if (j + 2 <= je + 1) nx = at_load(psi, psi_n, c, c_n, Nx, j + 2, ic);
         const double Jx = (double)at_flux(k, Z.f, Z.fr, Z.g, Z.gr, Z.c, Z.cr, P.f, P.fr, M.f, M.fr);
         const double Jxm = at_up(Jx);
         const double Jy = (double)at_yface(k, Z, P);
         if (own) {
             const double dc = -a.dt * ((Jx - Jxm) + (Jy - Jy_prev)) * a.idx;
             const int64_t row = (int64_t)j * Nx;
-            const double v = fmax(c_n[row + i] + dc, 1.0e-6);
+            const double v = fmax(Z.cn + dc, 1.0e-6);
             c_n[row + i] = v;
             if (i == 1) c_n[row] = v;
             if (i == Nx - 2) c_n[row + Nx - 1] = v;

@@ -407,7 +407,7 @@ TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tc
 namespace rs {
 
 // antitrapping pass of one run (the band routine is in rs_antitrap.hpp)
-constexpr int kAtBand = 32;
+constexpr int kAtBand = 64;
 
 template <typename T>
 __global__ __launch_bounds__(256) void antitrap_kernel(const T* psi, const T* psi_n, const double* c,
