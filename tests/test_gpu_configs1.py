@@ -1,0 +1,84 @@
+"""The 2048^2 coupled-T Ti-Nb configuration at its full length: 20 000 steps
+of the FP64 GPU path from the seeded planar front (init_config(21)) against
+the CPU oracle's state at step 20 000, held in
+tests/golden/configs1_2048_20000.npz (written by make_configs1_2048.py
+beside it; the oracle needs ~2 h on 8 cores for this run, the GPU a few
+seconds). The fixture keeps 8 full rows and a 12-column window around the
+tip column of psi and c, every 8th coarse-T row, integral figures and the
+tip row; tolerance 1e-10 relative to each field's largest magnitude.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T
+
+pytestmark = pytest.mark.gpu
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                       "configs1_2048_20000.npz")
+REL = 1e-10
+
+
+def say(*a):
+    print("[gpu-configs1]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def golden():
+    if not os.path.exists(FIXTURE):
+        pytest.skip("tests/golden/configs1_2048_20000.npz not generated")
+    with np.load(FIXTURE, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def test_configs1_20000_steps_vs_oracle(rs, golden):
+    g = golden
+    N, seed, steps = int(g["N"]), int(g["seed"]), int(g["steps"])
+    p = rs.config_params(N, True)
+    say(f"{N}^2 coupled T, FP64, {steps} steps, seed {seed}")
+    ctx = rs.Context(p)
+    ctx.init_config(seed)
+    ctx.step(steps)
+    k = ctx.constants()
+    assert k.step == int(g["step"])
+    assert k.i_left_end == int(g["i_left_end"])
+    assert k.i_offset_history == int(g["i_offset_history"])
+    assert k.l_offset_Vp == pytest.approx(float(g["l_offset_Vp"]), rel=1e-14)
+    psi = ctx.download(RS_FIELD_PSI)
+    c = ctx.download(RS_FIELD_C)
+    T = ctx.download(RS_FIELD_T)
+    rows = g["rows"]
+    i0 = int(g["win_i0"])
+    w = g["win_psi"].shape[1]
+    checks = (
+        ("psi rows", psi[rows, :], g["rows_psi"], float(g["max_abs_psi"])),
+        ("c rows", c[rows, :], g["rows_c"], float(g["max_abs_c"])),
+        ("psi tip window", psi[:, i0:i0 + w], g["win_psi"], float(g["max_abs_psi"])),
+        ("c tip window", c[:, i0:i0 + w], g["win_c"], float(g["max_abs_c"])),
+        ("T rows", T[::8, :], g["T_rows"], float(np.max(np.abs(g["T_rows"])))),
+    )
+    for name, got, ref, scale in checks:
+        assert got.shape == ref.shape, name
+        e = float(np.max(np.abs(got - ref))) / scale
+        say(name, "max rel err", e)
+        assert e <= REL, (name, e)
+    say("max |psi|", float(np.max(np.abs(psi))), "oracle", float(g["max_abs_psi"]))
+    assert float(np.max(np.abs(psi))) == pytest.approx(float(g["max_abs_psi"]), rel=REL)
+    for name, got, ref in (("sumsq psi", float(np.sum(psi * psi)), float(g["sumsq_psi"])),
+                           ("sumsq c", float(np.sum(c * c)), float(g["sumsq_c"])),
+                           ("sum c", float(np.sum(c)), float(g["sum_c"]))):
+        say(name, got, "oracle", ref)
+        assert got == pytest.approx(ref, rel=REL), name
+    m = ctx.solute_mass()
+    say("solute mass", m, "oracle", float(g["solute_mass"]))
+    assert m == pytest.approx(float(g["solute_mass"]), rel=1e-12)
+    assert ctx.tip_column() == int(g["tip_column"])
+    t = ctx.tip()
+    assert (t["i"], t["j"]) == (int(g["tip_i"]), int(g["tip_j"]))
+    assert t["x"] == pytest.approx(float(g["tip_x"]), abs=1e-8)
+    assert t["y"] == pytest.approx(float(g["tip_y"]), abs=1e-8)
+    ctx.close()
