@@ -71,6 +71,7 @@ int build_tables(rs_batch* b) {
     b->h_dyn.resize(n);
     std::vector<int> item0(n + 1, 0), cell0(n + 1, 0);
     bool ident = true, at_any = false;
+    int nsub = 0;
     for (int r = 0; r < n; ++r) {
         rs_ctx* c = b->runs[r];
         c->band_h = b->band_h[r];
@@ -92,6 +93,7 @@ int build_tables(rs_batch* b) {
         R.nbg = (R.a.nbands + b->L.wpb - 1) / b->L.wpb;
         R.tfc = p.ifTFC;
         R.ncT = p.ifTFC ? k.NxT * k.NyT : 0;
+        if (p.ifTFC) nsub = std::max(nsub, R.t.ratio_dt_dtT);
         R.Vp = p.Vp;
         R.Vp_rate = p.Vp_ramp_rate;
         R.ramp = p.if_Vp_ramp;
@@ -143,6 +145,7 @@ int build_tables(rs_batch* b) {
     L.cells = cell0[n];
     L.ident = ident;
     L.at_any = at_any ? 1 : 0;
+    L.nsub = nsub;
     const rs_ctx* c0 = b->runs[0];
     L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
     if (L.surr) L.surr = 1 + b->h_runs[0].a.s_fmt;
@@ -338,12 +341,6 @@ int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out) {
             for (rs_ctx* q : b->runs) rs_destroy(q);
             delete b;
             return rc;
-        }
-        if (c->p.ifTFC && c->k.ratio_dt_dtT != 1) {
-            rs_destroy(c);
-            for (rs_ctx* q : b->runs) rs_destroy(q);
-            delete b;
-            return RS_ERR_ARG;  // one coarse-T sub-step per step in the captured sequence
         }
         b->runs.push_back(c);
         const auto key = std::make_pair(params[r].precision, std::make_pair(params[r].Nx, params[r].Ny));

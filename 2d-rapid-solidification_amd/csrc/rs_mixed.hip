@@ -702,15 +702,20 @@ __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaun
     }
 }
 
-__global__ __launch_bounds__(256) void batch_temperature_kernel(BatchLaunch L) {
+// coarse-T sub-step `sub` of every coupled run that takes that many
+// (ratio_dt_dtT per step, fun.cu:496): sub-step s reads parity tcur ^ (s & 1)
+// and writes the other, as the host loop's buffer swaps do
+__global__ __launch_bounds__(256) void batch_temperature_kernel(BatchLaunch L, int sub) {
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < L.cells; q += gridDim.x * blockDim.x) {
         const int r = batch_run_of(L.cell0, L.nruns, q);
         const BatchRun& R = L.runs[r];
+        if (sub >= R.t.ratio_dt_dtT) continue;
         const BatchDyn d = L.dyn[r];
+        const int src = d.tcur ^ (sub & 1);
         TempArgs t = R.t;
-        t.T = R.T[d.tcur];
-        t.Tn = R.T[d.tcur ^ 1];
-        t.T32n = R.T32[d.tcur ^ 1];
+        t.T = R.T[src];
+        t.Tn = R.T[src ^ 1];
+        t.T32n = R.T32[src ^ 1];
         t.vadv = batch_vp(R, d.step * R.tau0 * R.dt) * R.tau0 / R.SW0;
         const int local = q - L.cell0[r];
         temperature_cell(t, local % t.NxT, local / t.NxT);
@@ -724,7 +729,7 @@ __global__ void batch_advance_kernel(BatchLaunch L) {
     const BatchRun& R = L.runs[r];
     BatchDyn d = L.dyn[r];
     if (R.tfc) {
-        d.tcur ^= 1;
+        d.tcur ^= R.t.ratio_dt_dtT & 1;
         d.ile_interp = R.t.ile;
     }
     d.cur ^= 1;
@@ -779,8 +784,8 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (L.cells > 0) {
-        hipLaunchKernelGGL(batch_temperature_kernel, dim3(L.T_blocks), dim3(256), 0, s, L);
+    for (int sub = 0; L.cells > 0 && sub < L.nsub; ++sub) {
+        hipLaunchKernelGGL(batch_temperature_kernel, dim3(L.T_blocks), dim3(256), 0, s, L, sub);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

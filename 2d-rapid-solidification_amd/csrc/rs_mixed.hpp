@@ -96,9 +96,10 @@ struct BatchLaunch {
     bool ident;
     int surr;           // 0 exact, else 1 + the surrogate's activation format
     int at_any;         // some run has the antitrapping correction on
+    int nsub;           // coarse-T sub-steps per step: the largest ratio_dt_dtT of a coupled run
 };
 // one step of every run: fused step kernel, antitrapping pass (runs with
-// it on), coarse T kernel, state advance
+// it on), coarse T kernel once per sub-step, state advance
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);
 // blocks of the step kernel resident per CU for waves-per-block wpb
 int batch_step_resident(const BatchLaunch& L);

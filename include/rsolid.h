@@ -384,10 +384,12 @@ int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset);
  * (RS_PREC_MIXED or RS_PREC_SURROGATE) -- advance together: each step is one
  * persistent launch of the fused step kernel over the work list of all runs
  * (run, column tile, band group), the antitrapping pass of the runs that
- * have it on, one coarse-T launch and a state-advance launch, and the step
- * sequence is captured once in a HIP graph and replayed. Pull-backs (fun.cu:448) split the replay into segments and run
+ * have it on, one coarse-T launch per sub-step (the largest ratio_dt_dtT of
+ * the coupled runs; each run takes its own number) and a state-advance
+ * launch, and the step sequence is captured once in a HIP graph and
+ * replayed. Pull-backs (fun.cu:448) split the replay into segments and run
  * on the host between them. Results are bitwise those of the runs stepped
- * one by one with rs_step. Coupled-T runs need ratio_dt_dtT == 1. */
+ * one by one with rs_step. */
 typedef struct rs_batch rs_batch;
 
 int rs_batch_create(const rs_params* params, int32_t n, rs_batch** out);

@@ -88,6 +88,27 @@ def test_batch_antitrapping_matches_single_runs(rs):
     b.close()
 
 
+def test_batch_coarse_T_substeps_match_single_runs(rs):
+    """Coupled runs taking 1, 2, 3 and 5 coarse-T sub-steps per step
+    (alpha = 100, 200, 300, 500 D_l, so ratio_dt_dtT = ceil(dt / dtT)) and a
+    frozen-T run in one batch, against rs_step, pull-backs on."""
+    params = [rs.config_params(n, True, alpha_ratio=ar, precision=RS_PREC_MIXED)
+              for n, ar in ((256, 100.0), (256, 200.0), (320, 300.0), (256, 500.0))]
+    params.append(rs.config_params(256, False, precision=RS_PREC_MIXED))
+    assert [rs.derive(p).ratio_dt_dtT for p in params[:4]] == [1, 2, 3, 5]
+    seeds = [41, 42, 43, 44, 45]
+    b = rs.Batch(params)
+    for run, s in zip(b.runs, seeds):
+        run.init_config(s)
+        k = run.constants()
+        k.step_pull_back = 120
+        run.set_constants(k)
+    say("coarse-T sub-steps batch")
+    b.step(250)
+    _check(b, _singles(rs, params, seeds, 250, 120))
+    b.close()
+
+
 def test_batch_tune_keeps_state_and_results(rs):
     params = [rs.ensemble_params(n, dT, v, RS_PREC_MIXED) for n, dT, v in SPECS]
     seeds = [21, 22, 23, 24]
