@@ -67,9 +67,12 @@ __device__ __forceinline__ Cell make_cell(const StepArgsF64& a, double psi, doub
 
 __global__ __launch_bounds__(256) void step_fp64_kernel(StepArgsF64 a) {
     const int lane = threadIdx.x;
-    const int band = blockIdx.y * blockDim.y + threadIdx.y;
+    // logical workgroup (column tile, band group) in XCD order
+    const int ntx = gridDim.x;
+    const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
+    const int band = (g / ntx) * blockDim.y + threadIdx.y;
     if (band >= a.nbands) return;  // whole wavefront leaves together
-    const int i = blockIdx.x * kTileOut + lane;
+    const int i = (g % ntx) * kTileOut + lane;
     const int ic = i < a.Nx ? i : a.Nx - 1;  // loads of lanes past the grid
     const bool own = lane >= 1 && lane <= kTileOut && i <= a.Nx - 2;
     const int jb = 1 + band * a.band_h;
@@ -412,10 +415,13 @@ constexpr int kAtBand = 64;
 template <typename T>
 __global__ __launch_bounds__(256) void antitrap_kernel(const T* psi, const T* psi_n, const double* c,
                                                        double* c_n, int Nx, int Ny, AtArgs a) {
-    const int band = blockIdx.y * blockDim.y + threadIdx.y;
+    // logical workgroup (column tile, band group) in XCD order
+    const int ntx = gridDim.x;
+    const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
+    const int band = (g / ntx) * blockDim.y + threadIdx.y;
     const int jb = 1 + band * kAtBand;
     if (jb > Ny - 2) return;
-    antitrap_band<T>(psi, psi_n, c, c_n, Nx, Ny, a, blockIdx.x, jb, min(jb + kAtBand - 1, Ny - 2));
+    antitrap_band<T>(psi, psi_n, c, c_n, Nx, Ny, a, g % ntx, jb, min(jb + kAtBand - 1, Ny - 2));
 }
 
 template <typename P>

@@ -10,6 +10,19 @@ namespace rs {
 constexpr int kWave = 64;
 constexpr int kTileOut = kWave - 2;
 
+// The dispatcher hands workgroup w (in launch order) to XCD w % 8, each XCD
+// with its own L2. xcd_order(w, n) is the position of w in the order that
+// gives each XCD a contiguous run of logical indices (a bijection of
+// [0, n)), so workgroups with neighbouring logical indices -- neighbouring
+// column tiles, which read the same 128-byte lines at their shared edges --
+// share an L2 and run at about the same time.
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int xcd_order(int w, int n) {
+    const int xcd = w % kXcds, k = w / kXcds;
+    const int full = n / kXcds, rem = n % kXcds;
+    return xcd * full + (xcd < rem ? xcd : rem) + k;
+}
+
 // Fine-grid temperature source: coupled coarse grid (bilinear, fun.cu:1580)
 // or frozen gradient (get_delta, fun.cu:1610).
 struct TempSrc {

@@ -597,8 +597,11 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
                         blockDim.x * blockDim.y);
     __syncthreads();
     unsigned nfb = 0, nall = 0;
-    step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), blockIdx.x,
-                           blockIdx.y * blockDim.y + threadIdx.y, nfb, nall);
+    // logical workgroup (column tile, band group) in XCD order
+    const int ntx = gridDim.x;
+    const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
+    step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
+                                      (g / ntx) * blockDim.y + threadIdx.y, nfb, nall);
     if constexpr (SURR != 0) {
         if (threadIdx.x == 0 && nall > 0) {
             atomicAdd(&a.fb_count[0], (unsigned long long)nfb);

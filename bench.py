@@ -349,6 +349,8 @@ def main() -> int:
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--antitrapping", action="store_true",
+                    help="with --only: also time the antitrapping extension section")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
                          "(byte-counter calibration for rocprofv3 --pmc)")
@@ -416,7 +418,7 @@ def main() -> int:
                             "max_residual_trusted": r["train"]["max_res_trusted"]}
         if "loaded_from" in r["train"]:
             out["surrogate"]["weights_loaded_from"] = r["train"]["loaded_from"]
-    if not a.only:
+    if not a.only or a.antitrapping:
         out["antitrapping"] = run_antitrapping(a.grid, 300, 20)
     if a.ensemble_runs > 0 and not a.only:
         out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
