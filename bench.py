@@ -81,15 +81,15 @@ def _cpu_model() -> str:
 def cpu_baseline(grid: int, seconds: float):
     """The CPU oracle (C, FP64, OpenMP over rows) on the host cores, timed on
     a bounded sample of the same workload: the first steps of the same grid,
-    with up to 16 threads, then a shorter sample on one thread."""
+    with up to 16 threads, then a shorter sample on one thread; and shorter
+    samples of the 256^2 and 2048^2 coupled configurations."""
     import oracle
     import rsolid
     cores = min(16, os.cpu_count() or 1)
-    p = rsolid.config_params(grid, tfc=True)
 
-    def sample(threads: int, secs: float):
+    def sample(n_grid: int, threads: int, secs: float):
         oracle.set_threads(threads)
-        st = oracle.State(p)
+        st = oracle.State(rsolid.config_params(n_grid, tfc=True))
         st.init_config(5)
         st.step(1)                     # step 0 (pull-back) outside the timing
         n, t0 = 0, time.perf_counter()
@@ -100,15 +100,22 @@ def cpu_baseline(grid: int, seconds: float):
             if el >= secs:
                 break
         st.close()
-        return grid * grid * n / el / 1e6, n, el
+        return n_grid * n_grid * n / el / 1e6, n, el
 
-    v, n, el = sample(cores, seconds)
-    v1, n1, el1 = sample(1, seconds / 3)
+    v, n, el = sample(grid, cores, seconds)
+    v1, n1, el1 = sample(grid, 1, seconds / 3)
+    grids = {}
+    for g in (256, 2048):
+        if g != grid:
+            vg, ng, _ = sample(g, cores, seconds / 6)
+            vg1, ng1, _ = sample(g, 1, seconds / 12)
+            grids[str(g)] = {"value": round(vg, 3), "value_1thread": round(vg1, 3),
+                             "steps": ng, "steps_1thread": ng1}
     oracle.set_threads(cores)
     return {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"CPU FP64 oracle, {grid}x{grid} coupled Ti-Nb, {n} steps ({el:.1f} s) on "
                       f"{cores} threads; {n1} steps ({el1:.1f} s) on 1 thread",
-            "value_1thread": round(v1, 3), "cpu_model": _cpu_model(),
+            "value_1thread": round(v1, 3), "grids": grids, "cpu_model": _cpu_model(),
             "nproc": os.cpu_count(), "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
