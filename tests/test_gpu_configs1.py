@@ -5,7 +5,10 @@ tests/golden/configs1_2048_20000.npz (written by make_configs1_2048.py
 beside it; the oracle needs ~2 h on 8 cores for this run, the GPU a few
 seconds). The fixture keeps 8 full rows and a 12-column window around the
 tip column of psi and c, every 8th coarse-T row, integral figures and the
-tip row; tolerance 1e-10 relative to each field's largest magnitude.
+tip row; tolerance 1e-10 relative to each field's largest magnitude for
+the FP64 path. The FP32 path is held to the precision targets over the same
+run (tip advance within 1%, solute mass within 1e-10) and to stated field
+bounds.
 """
 import os
 import sys
@@ -13,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED
 
 pytestmark = pytest.mark.gpu
 
@@ -81,4 +84,47 @@ def test_configs1_20000_steps_vs_oracle(rs, golden):
     assert (t["i"], t["j"]) == (int(g["tip_i"]), int(g["tip_j"]))
     assert t["x"] == pytest.approx(float(g["tip_x"]), abs=1e-8)
     assert t["y"] == pytest.approx(float(g["tip_y"]), abs=1e-8)
+    ctx.close()
+
+
+def test_configs1_fp32_path_20000_steps(rs, golden):
+    """The FP32 path (FP32 psi/phi and fine T, FP64 c) over the same 20 000
+    steps against the oracle's state: the tip advance within 1% of the
+    oracle's and the solute mass within 1e-10 (the precision targets), the
+    psi / c / T deviations reported and held to 1e-4 absolute (psi on the
+    sampled rows and the tip window), 1e-6 relative (c) and 1e-7 relative
+    (T)."""
+    g = golden
+    N, seed, steps = int(g["N"]), int(g["seed"]), int(g["steps"])
+    p = rs.config_params(N, True, precision=RS_PREC_MIXED)
+    say(f"{N}^2 coupled T, FP32 path, {steps} steps, seed {seed}")
+    ctx = rs.Context(p)
+    ctx.init_config(seed)
+    x0 = ctx.constants().i_offset_history + ctx.tip()["x"]
+    ctx.step(steps)
+    k = ctx.constants()
+    assert k.step == int(g["step"])
+    psi = ctx.download(RS_FIELD_PSI)
+    c = ctx.download(RS_FIELD_C)
+    T = ctx.download(RS_FIELD_T)
+    rows = g["rows"]
+    i0 = int(g["win_i0"])
+    w = g["win_psi"].shape[1]
+    epsi = max(float(np.max(np.abs(psi[rows, :] - g["rows_psi"]))),
+               float(np.max(np.abs(psi[:, i0:i0 + w] - g["win_psi"]))))
+    ec = max(float(np.max(np.abs(c[rows, :] - g["rows_c"]))),
+             float(np.max(np.abs(c[:, i0:i0 + w] - g["win_c"])))) / float(g["max_abs_c"])
+    eT = float(np.max(np.abs(T[::8, :] - g["T_rows"]))) / float(np.max(np.abs(g["T_rows"])))
+    m = ctx.solute_mass()
+    dm = m / float(g["solute_mass"]) - 1.0
+    t = ctx.tip()
+    x1 = k.i_offset_history + t["x"]
+    xo = int(g["i_offset_history"]) + float(g["tip_x"])
+    say("psi Linf (sampled)", epsi, "c rel", ec, "T rel", eT, "mass rel", dm,
+        "tip advance", x1 - x0, "oracle", xo - x0)
+    assert abs((x1 - x0) - (xo - x0)) <= 0.01 * abs(xo - x0)
+    assert abs(dm) <= 1e-10
+    assert epsi <= 1e-4
+    assert ec <= 1e-6
+    assert eT <= 1e-7
     ctx.close()
