@@ -9,7 +9,7 @@ import math
 import numpy as np
 import pytest
 
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T
 
 
 # ---------------------------------------------------------------------------
@@ -223,6 +223,54 @@ def test_anisotropy_matches_flux_form(rs, oracle_mod):
         errs.append(e)
     assert errs[1] < 5e-3
     assert errs[1] < errs[0] / 2.5   # second-order convergence
+
+
+@pytest.mark.parametrize("preset", ["config", "reference"])
+def test_phi_step_is_allen_cahn_and_keeps_the_tanh_profile(rs, oracle_mod, preset):
+    """compute_phi + compute_psi (fun.cu:1143-1273) as a whole step. With no
+    anisotropy (eps1 = epk1 = 0: a_s = a_k = 1) and no driving force (v0 so
+    large that (Gl - Gs) g'(phi) / (2 hh v0) vanishes; v0 cancels from the
+    solute equation, whose denominator h0 v0 = R Tmelt) the update is
+    explicit Allen-Cahn, phi + dt (lap phi + phi - phi^3); on a field uniform
+    in y the 5-point (configuration) and the 9-point (reference) Laplacians
+    both reduce to the 1D second difference, evaluated here with numpy. The
+    planar profile -tanh(x / sqrt 2) solves phi'' + phi - phi^3 = 0, so one
+    step moves it by the second difference's truncation error,
+    dt dx^2 / 12 times the largest fourth derivative to leading order."""
+    if preset == "config":
+        p = rs.config_params(128, tfc=False)
+    else:
+        p = rs.reference_params(512, 32)
+    p.eps1 = 0.0
+    p.epk1 = 0.0
+    p.v0 = 1e200
+    st = oracle_mod.State(p)
+    if preset == "config":
+        st.init_config(1)
+    else:
+        st.init_reference()
+    k = st.constants()
+    k.step = 1                        # no step-0 pull-back
+    st.set_constants(k)
+    dx, dt = p.dx, k.dt
+    i0 = p.Nx // 2 + 0.37
+    x = (np.arange(p.Nx) - i0) * dx
+    phi = np.tile(-np.tanh(x / np.sqrt(2.0)), (p.Ny, 1))
+    st.put(RS_FIELD_PSI, phi)
+    st.put(RS_FIELD_PHI, phi)
+    st.step(1)
+    got = st.get(RS_FIELD_PSI)[1:-1, 1:-1]
+    d2 = (phi[:, 2:] - 2.0 * phi[:, 1:-1] + phi[:, :-2]) / (dx * dx)
+    p0 = phi[:, 1:-1]
+    want = (p0 + dt * (d2 + p0 - p0 ** 3))[1:-1]
+    assert np.max(np.abs(got - want)) <= 1e-13
+    # fourth derivative of -tanh(x / sqrt 2): tanh''''(u) / 4 with
+    # tanh''''(u) = 8 t (1 - t^2) (2 - 3 t^2), t = tanh u
+    t = np.tanh(np.linspace(-4.0, 4.0, 200001))
+    d4 = float(np.max(np.abs(8.0 * t * (1.0 - t * t) * (2.0 - 3.0 * t * t)))) / 4.0
+    bound = dt * dx * dx / 12.0 * d4
+    moved = float(np.max(np.abs(got - p0[1:-1])))
+    assert 0.5 * bound <= moved <= 1.25 * bound, (moved, bound)
 
 
 # ---------------------------------------------------------------------------
