@@ -90,6 +90,7 @@ void free_all(rs_ctx* ctx) {
     if (ctx->tipvals) (void)hipFree(ctx->tipvals);
     if (ctx->tiprow) (void)hipFree(ctx->tiprow);
     if (ctx->outi) (void)hipFree(ctx->outi);
+    if (ctx->finite_out) (void)hipFree(ctx->finite_out);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     for (int q = 0; q < 3; ++q)
         for (hipEvent_t e : ctx->ev[q]) (void)hipEventDestroy(e);
@@ -304,6 +305,41 @@ uint64_t splitmix64(uint64_t* s) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
+}
+
+// Finite-value sentinel over the current psi, c and (coupled T) coarse T:
+// RS_ERR_NONFINITE naming the first field with a NaN or Inf, its count and
+// the first such cell in row-major order.
+int finite_check(rs_ctx* ctx) {
+    if (!ctx->finite_out)
+        HIPCHK(ctx, hipMalloc(&ctx->finite_out, 6 * sizeof(unsigned long long)));
+    const unsigned long long init[6] = {0ull, ~0ull, 0ull, ~0ull, 0ull, ~0ull};
+    unsigned long long h[6];
+    HIPCHK(ctx, hipMemcpyAsync(ctx->finite_out, init, sizeof init, hipMemcpyHostToDevice,
+                               ctx->stream));
+    const int c0 = ctx->cur;
+    if (ctx->mixed)
+        HIPCHK(ctx, launch_nonfinite_f32(ctx->psi32[c0], ctx->n, ctx->finite_out, ctx->stream));
+    else
+        HIPCHK(ctx, launch_nonfinite(ctx->psi[c0], ctx->n, ctx->finite_out, ctx->stream));
+    HIPCHK(ctx, launch_nonfinite(ctx->c[c0], ctx->n, ctx->finite_out + 2, ctx->stream));
+    if (ctx->p.ifTFC)
+        HIPCHK(ctx, launch_nonfinite(ctx->T[ctx->tcur], ctx->nT, ctx->finite_out + 4, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->finite_out, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const char* names[3] = {"psi", "c", "coarse T"};
+    for (int f = 0; f < 3; ++f) {
+        if (h[2 * f] == 0) continue;
+        const int64_t w = f == 2 ? ctx->k.NxT_all : ctx->p.Nx;
+        const int64_t q = (int64_t)h[2 * f + 1];
+        char b[256];
+        snprintf(b, sizeof b,
+                 "non-finite value after step %lld: %s has %llu such cells, the first at "
+                 "(i=%lld, j=%lld)",
+                 (long long)ctx->k.step, names[f], h[2 * f], (long long)(q % w), (long long)(q / w));
+        return fail(ctx, RS_ERR_NONFINITE, b);
+    }
+    return RS_OK;
 }
 
 }  // namespace
@@ -634,6 +670,10 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
         ctx->cur ^= 1;
         k.l_offset_Vp += vp_ramp(p, step * tau0 * k.dt) * k.dt * tau0 / (p.S * p.W0);
         k.step = step + 1;
+        if (ctx->finite_every > 0 && ++ctx->finite_ctr % ctx->finite_every == 0) {
+            const int rc = finite_check(ctx);
+            if (rc) return rc;
+        }
     }
     return RS_OK;
 }
@@ -745,6 +785,13 @@ int rs_timing_enable(rs_ctx* ctx, int32_t every) {
     ctx->timing = every;
     ctx->timing_ctr = 0;
     for (int q = 0; q < 3; ++q) ctx->nev[q] = 0;
+    return RS_OK;
+}
+
+int rs_set_finite_check(rs_ctx* ctx, int32_t every) {
+    if (!ctx || every < 0) return RS_ERR_ARG;
+    ctx->finite_every = every;
+    ctx->finite_ctr = 0;
     return RS_OK;
 }
 

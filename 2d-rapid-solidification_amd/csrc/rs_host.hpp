@@ -73,6 +73,11 @@ struct rs_ctx {
     // [1] on every timing-th step (0: off)
     int timing = 0;
     int64_t timing_ctr = 0;
+    // finite-value sentinel: check every finite_every-th step (0: off);
+    // finite_out holds (count, first index) per checked field
+    int finite_every = 0;
+    int64_t finite_ctr = 0;
+    unsigned long long* finite_out = nullptr;
     std::vector<hipEvent_t> ev[3];
     int nev[3] = {0, 0, 0};
 };

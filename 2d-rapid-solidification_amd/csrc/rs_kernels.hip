@@ -409,6 +409,39 @@ TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tc
 
 namespace rs {
 
+// finite-value sentinel: count and first index of NaN / Inf values (the
+// count and the minimum do not depend on the order of the atomics)
+template <typename T>
+__global__ __launch_bounds__(256) void nonfinite_kernel(const T* f, int64_t n,
+                                                        unsigned long long* out) {
+    unsigned long long cnt = 0, first = ~0ull;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        if (!isfinite(f[q])) {
+            ++cnt;
+            if ((unsigned long long)q < first) first = (unsigned long long)q;
+        }
+    }
+    if (cnt) {
+        atomicAdd(&out[0], cnt);
+        atomicMin(&out[1], first);
+    }
+}
+
+template <typename T>
+static hipError_t launch_nf(const T* f, int64_t n, unsigned long long* out, hipStream_t s) {
+    const int64_t want = (n + 255) / 256;
+    const int blocks = (int)(want < 2048 ? (want > 0 ? want : 1) : 2048);
+    hipLaunchKernelGGL(nonfinite_kernel<T>, dim3(blocks), dim3(256), 0, s, f, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_nonfinite(const double* f, int64_t n, unsigned long long* out, hipStream_t s) {
+    return launch_nf(f, n, out, s);
+}
+hipError_t launch_nonfinite_f32(const float* f, int64_t n, unsigned long long* out, hipStream_t s) {
+    return launch_nf(f, n, out, s);
+}
+
 // antitrapping pass of one run (the band routine is in rs_antitrap.hpp)
 constexpr int kAtBand = 64;
 

@@ -138,6 +138,11 @@ hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* 
                        hipStream_t s);
 hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
 hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s);
+// number of non-finite values of f[0, n) and the smallest index holding one
+// (out[0] += count, out[1] = min(out[1], index); out must be initialised to
+// 0 and ~0)
+hipError_t launch_nonfinite(const double* f, int64_t n, unsigned long long* out, hipStream_t s);
+hipError_t launch_nonfinite_f32(const float* f, int64_t n, unsigned long long* out, hipStream_t s);
 // per-row data of the tip fit: irow[Ny], vals[8 Ny] (see precise_tip_rows)
 hipError_t launch_tip_detail(const double* psi, int Nx, int Ny, double pin, int* irow,
                              double* vals, hipStream_t s);

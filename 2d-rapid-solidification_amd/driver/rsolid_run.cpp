@@ -12,7 +12,9 @@
 // orientation, degrees), di dj (output stride),
 // state=1 (also write the exact state file for a bit-exact restart),
 // load_state=FILE (restart from such a file instead of the checkpoints),
-// pull_back=N (pull-back cadence in steps instead of the derived one).
+// pull_back=N (pull-back cadence in steps instead of the derived one),
+// finite_check=N (stop with an error naming the field and cell when psi, c
+// or the coarse T holds a NaN or Inf, checked every N steps).
 //
 // Files (under path_input, as the reference writes them): data/c_N.txt and
 // data/phi0_N.txt every step_fields, tip.txt every step_tip,
@@ -221,6 +223,8 @@ int main(int argc, char** argv) {
         kp.step_pull_back = (int32_t)argl("pull_back", 0);
         CHECK(r.ctx, rs_set_constants(r.ctx, &kp));
     }
+    if (argl("finite_check", 0) > 0)
+        CHECK(r.ctx, rs_set_finite_check(r.ctx, (int32_t)argl("finite_check", 0)));
     char dev[256] = "";
     rs_device_name(dev, sizeof dev);
     printf("rsolid_run on %s, %d x %d, precision %s\n", dev, r.p.Nx, r.p.Ny, prec.c_str());

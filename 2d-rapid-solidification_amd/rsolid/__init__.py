@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 from .abi import (  # noqa: F401  (re-exported)
-    EXPORTED, RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_STATE, RS_FIELD_C,
+    EXPORTED, RS_ACT_BF16, RS_ACT_BF16_SPLIT, RS_ACT_FP16, RS_ACT_FP16_PACKED, RS_ERR_ARG, RS_ERR_DOMAIN, RS_ERR_HIP, RS_ERR_NOMEM, RS_ERR_NONFINITE, RS_ERR_STATE, RS_FIELD_C,
     RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE, RS_OK, RS_PREC_FP64, RS_PREC_MIXED,
     RS_PREC_SURROGATE, RS_SURR_NMETA, RS_SURR_NPARAM, RS_SURR_NREPORT, RsConstants, RsParams,
     RsSurrogateOpts, RsTipRow,
@@ -58,6 +58,7 @@ def _load() -> C.CDLL:
         "rs_tip": (C.c_int, [vp, P(RsTipRow)]),
         "rs_sync": (C.c_int, [vp]),
         "rs_timing_enable": (C.c_int, [vp, C.c_int32]),
+        "rs_set_finite_check": (C.c_int, [vp, C.c_int32]),
         "rs_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "rs_calibrate_copy": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double)]),
         "rs_context_info": (C.c_int, [vp, P(C.c_double), C.c_int32]),
@@ -291,6 +292,12 @@ class Context:
     def timing_enable(self, on: bool = True, every: int = 1) -> None:
         """HIP-event timing of the kernels of every `every`-th step."""
         self._chk(lib.rs_timing_enable(self._h, every if on else 0), "rs_timing_enable")
+
+    def finite_check(self, every: int = 1) -> None:
+        """Check psi, c and the coarse T for NaN / Inf after every `every`-th
+        step (0: off); a step that finds one raises RsError
+        (RS_ERR_NONFINITE) naming the step, field and first cell."""
+        self._chk(lib.rs_set_finite_check(self._h, every), "rs_set_finite_check")
 
     def info(self) -> dict:
         v = (C.c_double * 6)()

@@ -13,6 +13,7 @@ RS_ERR_HIP = -2
 RS_ERR_DOMAIN = -3
 RS_ERR_NOMEM = -4
 RS_ERR_STATE = -5
+RS_ERR_NONFINITE = -6
 
 RS_PREC_FP64 = 0
 RS_PREC_MIXED = 1
@@ -105,6 +106,7 @@ EXPORTED = [
     "rs_last_error", "rs_device_name", "rs_init_reference", "rs_init_config", "rs_upload",
     "rs_download", "rs_get_constants", "rs_set_constants", "rs_step", "rs_tip_column",
     "rs_pull_back", "rs_solute_mass", "rs_tip", "rs_sync", "rs_timing_enable", "rs_timing",
+    "rs_set_finite_check",
     "rs_calibrate_copy", "rs_context_info", "rs_surrogate_defaults", "rs_surrogate_train",
     "rs_surrogate_get", "rs_surrogate_set", "rs_surrogate_grad", "rs_surrogate_eval",
     "rs_surrogate_stats", "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error",

@@ -42,7 +42,8 @@ enum {
     RS_ERR_HIP = -2,      /* a HIP runtime call failed                       */
     RS_ERR_DOMAIN = -3,   /* physical/geometric domain violated (T window)   */
     RS_ERR_NOMEM = -4,    /* device allocation failed                        */
-    RS_ERR_STATE = -5     /* call not valid in the context's current state   */
+    RS_ERR_STATE = -5,    /* call not valid in the context's current state   */
+    RS_ERR_NONFINITE = -6 /* finite-value check found a NaN or Inf (opt-in)   */
 };
 
 /* ---- precision / thermodynamics paths ---------------------------------- */
@@ -252,6 +253,16 @@ int rs_sync(rs_ctx* ctx);
  * on), 1 = coarse T kernel(s). */
 int rs_timing_enable(rs_ctx* ctx, int32_t every);
 int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
+
+/* Finite-value sentinel (no counterpart in the reference, where a NaN or
+ * Inf propagates silently through every later step): after
+ * rs_set_finite_check(ctx, every) with every >= 1, rs_step checks psi, c and
+ * (coupled T) the coarse T after every every-th step and stops at the first
+ * check that finds a non-finite value, returning RS_ERR_NONFINITE;
+ * rs_last_error names the step, the field, the number of such cells and the
+ * first one in row-major order. The step's results are kept. Each check
+ * reads the fields once and waits for the stream. 0 (default) turns it off. */
+int rs_set_finite_check(rs_ctx* ctx, int32_t every);
 
 /* Measurement helper with no counterpart in the reference: launches a plain
  * FP64 copy kernel (n doubles read, n written, n <= Nx*Ny) reps times on
