@@ -11,7 +11,8 @@ import sys
 import numpy as np
 import pytest
 
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE
+from rsolid.abi import (RS_FIELD_C, RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T, RS_FIELD_TFINE,
+                        RS_PREC_MIXED)
 
 pytestmark = pytest.mark.gpu
 
@@ -205,6 +206,35 @@ def test_full_size_properties(rs):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
     assert outs[0][2] == outs[1][2]
+
+
+def test_full_size_matches_oracle(rs, oracle_mod):
+    """The 4096^2 coupled-T workload of the bench line itself, 20 steps from
+    the seeded front, against the oracle at the run parity bound (1e-10
+    relative per field, integer state equal), and the FP32 path from the
+    same state within its stated bounds (psi 2e-6 absolute, c 1e-7
+    relative)."""
+    p = rs.config_params(4096, True)
+    say("4096^2 coupled, 20 steps vs oracle")
+    st = oracle_mod.State(p)
+    st.init_config(5)
+    st.step(20)
+    ctx = rs.Context(p)
+    ctx.init_config(5)
+    ctx.step(20)
+    _compare(ctx, st, True, "4096@20")
+    ctx.close()
+    pm = rs.config_params(4096, True, precision=RS_PREC_MIXED)
+    cm = rs.Context(pm)
+    cm.init_config(5)
+    cm.step(20)
+    epsi = float(np.max(np.abs(cm.download(RS_FIELD_PSI) - st.get(RS_FIELD_PSI))))
+    ec = _relerr(cm.download(RS_FIELD_C), st.get(RS_FIELD_C))
+    say("4096 FP32 path psi Linf", epsi, "c rel", ec)
+    assert epsi <= 2e-6
+    assert ec <= 1e-7
+    cm.close()
+    st.close()
 
 
 @pytest.mark.parametrize("v_ratio", [0.1, 2.0])
