@@ -150,12 +150,19 @@ StepArgsF64 step_args(rs_ctx* ctx) {
     return a;
 }
 
+// A shift that would move the fine window off the coarse T sample is
+// rejected before anything changes (the context keeps its state).
 int do_pull_back(rs_ctx* ctx, int off) {
     rs_constants& k = ctx->k;
+    if (ctx->p.ifTFC) {
+        const int ile = k.i_left_end_initial + (k.i_offset_history + off) -
+                        (int)(k.l_offset_Vp / ctx->p.dx);
+        const int rc = check_window(ctx, ile);
+        if (rc) return rc;
+        k.i_left_end = ile;
+    }
     k.i_offset = off;
     k.i_offset_history += off;
-    if (ctx->p.ifTFC)
-        k.i_left_end = k.i_left_end_initial + k.i_offset_history - (int)(k.l_offset_Vp / ctx->p.dx);
     const int c0 = ctx->cur;
     if (ctx->mixed)
         HIPCHK(ctx, launch_pull_back_f32(ctx->psi32[c0], ctx->c[c0], ctx->psi32[c0 ^ 1],
@@ -165,7 +172,7 @@ int do_pull_back(rs_ctx* ctx, int off) {
         HIPCHK(ctx, launch_pull_back(ctx->psi[c0], ctx->c[c0], ctx->psi[c0 ^ 1], ctx->c[c0 ^ 1],
                                      ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf, ctx->stream));
     ctx->cur ^= 1;
-    return check_window(ctx, k.i_left_end);
+    return RS_OK;
 }
 
 int do_tip_column(rs_ctx* ctx, int* it) {

@@ -62,3 +62,41 @@ def test_bad_params_rejected_without_gpu(rs):
     h = C.c_void_p()
     assert rs.lib.rs_create(C.byref(p), C.byref(h)) == rs.RS_ERR_ARG
     assert rs.lib.rs_params_reference(C.byref(p), 100, 100) == rs.RS_ERR_ARG
+
+
+def _cfg(rs, **kw):
+    p = rs.config_params(256, True)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("case", ["grid_7", "coupled_Ny_21", "ratio_5", "precision_9",
+                                  "tip_outside", "antitrap_k0", "mixed_dx_fraction", "S_0"])
+def test_create_validates_every_parameter_rule(rs, case):
+    """rs_create rejects each parameter set its validation rules out with
+    RS_ERR_ARG before any device call (so this runs without a GPU)."""
+    make = {
+        "grid_7": lambda: _cfg(rs, Nx=7),
+        "coupled_Ny_21": lambda: _cfg(rs, Ny=21),
+        "ratio_5": lambda: _cfg(rs, ratio_dxT_dx=5),
+        "precision_9": lambda: _cfg(rs, precision=9),
+        "tip_outside": lambda: _cfg(rs, i_tip_target=255),
+        "antitrap_k0": lambda: _cfg(rs, at_coef=0.35, k_part=0.0),
+        "mixed_dx_fraction": lambda: _cfg(rs, precision=rs.RS_PREC_MIXED, dx=0.75),
+        "S_0": lambda: _cfg(rs, S=0),
+    }
+    h = C.c_void_p()
+    assert rs.lib.rs_create(C.byref(make[case]()), C.byref(h)) == rs.RS_ERR_ARG
+    assert not h.value
+
+
+def test_create_control_config_passes_validation(rs):
+    """The unmodified configuration the rejection cases start from passes
+    validation: without a GPU rs_create then fails at the device
+    (RS_ERR_HIP), with one it succeeds."""
+    h = C.c_void_p()
+    rc = rs.lib.rs_create(C.byref(_cfg(rs)), C.byref(h))
+    assert rc != rs.RS_ERR_ARG
+    if rc == rs.RS_OK:
+        rs.lib.rs_destroy(h)
