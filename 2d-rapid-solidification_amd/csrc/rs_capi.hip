@@ -14,6 +14,14 @@ using namespace rs;
 
 namespace {
 
+// why the calling thread's last rs_create failed (rs_last_error(NULL))
+thread_local std::string g_create_err;
+
+int create_fail(int code, const std::string& why) {
+    g_create_err = why;
+    return code;
+}
+
 int fail(rs_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
     if (ctx) {
         char buf[512];
@@ -369,13 +377,13 @@ int rs_abi_info(int32_t* s) {
 }
 
 int rs_create(const rs_params* p, rs_ctx** out) {
-    if (!p || !out) return RS_ERR_ARG;
+    if (!p || !out) return create_fail(RS_ERR_ARG, "null argument");
     *out = nullptr;
     std::string why;
     int rc = check_params(*p, &why);
-    if (rc) return rc;
+    if (rc) return create_fail(rc, why);
     rs_ctx* ctx = new (std::nothrow) rs_ctx;
-    if (!ctx) return RS_ERR_NOMEM;
+    if (!ctx) return create_fail(RS_ERR_NOMEM, "host allocation failed");
     ctx->p = *p;
     rs_derive(p, &ctx->k);
     ctx->d = derive_constants(*p, ctx->k.dt);
@@ -395,7 +403,7 @@ int rs_create(const rs_params* p, rs_ctx** out) {
         (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
         free_all(ctx);
         delete ctx;
-        return RS_ERR_HIP;
+        return create_fail(RS_ERR_HIP, std::string("no usable HIP device: ") + hipGetErrorString(e));
     }
     const size_t fb = sizeof(double) * (size_t)ctx->n;
     // coarse T: NxT_all x NyT_all plus one spare row (the bilinear read of
@@ -432,7 +440,7 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     if (!ok) {
         free_all(ctx);
         delete ctx;
-        return RS_ERR_NOMEM;
+        return create_fail(RS_ERR_NOMEM, "device allocation of the fields failed");
     }
     // defined contents everywhere (ghosts, padding, unused latent-heat rows)
     ok = hipMemsetAsync(ctx->c[0], 0, fb, ctx->stream) == hipSuccess &&
@@ -444,8 +452,9 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     if (!ok) {
         free_all(ctx);
         delete ctx;
-        return RS_ERR_HIP;
+        return create_fail(RS_ERR_HIP, "clearing the new fields failed");
     }
+    g_create_err.clear();
     *out = ctx;
     return RS_OK;
 }
@@ -468,7 +477,10 @@ int rs_destroy(rs_ctx* ctx) {
     return RS_OK;
 }
 
-const char* rs_last_error(const rs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* rs_last_error(const rs_ctx* ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_create_err.empty() ? "null context" : g_create_err.c_str();
+}
 
 int rs_device_name(char* buf, int32_t len) {
     if (!buf || len < 1) return RS_ERR_ARG;

@@ -205,7 +205,7 @@ class Context:
         h = C.c_void_p()
         rc = lib.rs_create(C.byref(self.p), C.byref(h))
         if rc:
-            raise RsError(rc, "rs_create")
+            raise RsError(rc, f"rs_create: {lib.rs_last_error(None).decode()}")
         self._h = h
         self.k0 = derive(self.p)
 

@@ -217,7 +217,9 @@ int rs_destroy(rs_ctx* ctx);
  * out[1] = Tref, out[2] = window low (K), out[3] = window high (K),
  * out[4] = band height (rows per wavefront), out[5] = waves per block. */
 int rs_context_info(rs_ctx* ctx, double* out, int32_t n);
-const char* rs_last_error(const rs_ctx* ctx);   /* never NULL */
+/* never NULL; with ctx NULL: why the calling thread's last rs_create failed
+ * (which rule its parameters broke, or the device error) */
+const char* rs_last_error(const rs_ctx* ctx);
 int rs_device_name(char* buf, int32_t len);
 
 /* Initial conditions: initial + initial_temperature + interpolate_temperature

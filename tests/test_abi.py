@@ -100,3 +100,13 @@ def test_create_control_config_passes_validation(rs):
     assert rc != rs.RS_ERR_ARG
     if rc == rs.RS_OK:
         rs.lib.rs_destroy(h)
+
+
+def test_create_failure_reason_is_kept(rs):
+    """rs_last_error(NULL) gives the reason of the calling thread's last
+    rs_create failure: the broken rule."""
+    h = C.c_void_p()
+    assert rs.lib.rs_create(C.byref(_cfg(rs, ratio_dxT_dx=5)), C.byref(h)) == rs.RS_ERR_ARG
+    assert b"ratio_dxT_dx must be 10" in rs.lib.rs_last_error(None)
+    assert rs.lib.rs_create(C.byref(_cfg(rs, Ny=21)), C.byref(h)) == rs.RS_ERR_ARG
+    assert b"coupled T needs Ny >= 22" in rs.lib.rs_last_error(None)
