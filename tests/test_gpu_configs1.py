@@ -16,7 +16,7 @@ import sys
 import numpy as np
 import pytest
 
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED, RS_PREC_SURROGATE
 
 pytestmark = pytest.mark.gpu
 
@@ -127,4 +127,46 @@ def test_configs1_fp32_path_20000_steps(rs, golden):
     assert epsi <= 1e-4
     assert ec <= 1e-6
     assert eT <= 1e-7
+    ctx.close()
+
+
+def test_configs1_surrogate_path_20000_steps(rs, golden):
+    """The surrogate path (network trained with the default options) over
+    the same 20 000 steps: the tip advance within 1% of the oracle's (the
+    tip-velocity target) and the fields within the surrogate step test's
+    bounds (psi 2e-2 absolute, c 2e-3 relative). Its solute-mass difference
+    is held to 1e-8, a regression bound: the path does not meet the 1e-10
+    drift target (the edge fluxes of the reference's ghost scheme respond
+    to the network's field perturbations)."""
+    g = golden
+    N, seed, steps = int(g["N"]), int(g["seed"]), int(g["steps"])
+    p = rs.config_params(N, True, precision=RS_PREC_SURROGATE)
+    say(f"{N}^2 coupled T, surrogate path, {steps} steps, seed {seed}")
+    ctx = rs.Context(p)
+    ctx.init_config(seed)
+    rep = ctx.surrogate_train()
+    say("trusted bins", rep["trusted_frac"], "window", rep["T_lo"], rep["T_hi"])
+    x0 = ctx.constants().i_offset_history + ctx.tip()["x"]
+    ctx.surrogate_stats(reset=True)
+    ctx.step(steps)
+    st = ctx.surrogate_stats()
+    k = ctx.constants()
+    psi = ctx.download(RS_FIELD_PSI)
+    c = ctx.download(RS_FIELD_C)
+    rows = g["rows"]
+    i0 = int(g["win_i0"])
+    w = g["win_psi"].shape[1]
+    epsi = max(float(np.max(np.abs(psi[rows, :] - g["rows_psi"]))),
+               float(np.max(np.abs(psi[:, i0:i0 + w] - g["win_psi"]))))
+    ec = max(float(np.max(np.abs(c[rows, :] - g["rows_c"]))),
+             float(np.max(np.abs(c[:, i0:i0 + w] - g["win_c"])))) / float(g["max_abs_c"])
+    dm = ctx.solute_mass() / float(g["solute_mass"]) - 1.0
+    x1 = k.i_offset_history + ctx.tip()["x"]
+    xo = int(g["i_offset_history"]) + float(g["tip_x"])
+    say("fallback", st["fallback_frac"], "psi Linf (sampled)", epsi, "c rel", ec, "mass rel", dm,
+        "tip advance", x1 - x0, "oracle", xo - x0)
+    assert abs((x1 - x0) - (xo - x0)) <= 0.01 * abs(xo - x0)
+    assert epsi <= 2e-2
+    assert ec <= 2e-3
+    assert abs(dm) <= 1e-8
     ctx.close()
