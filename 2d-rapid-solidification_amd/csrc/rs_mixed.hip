@@ -130,13 +130,6 @@ __device__ __forceinline__ float buf_ldf(Rsrc r, uint32_t voff, uint32_t soff) {
 __device__ __forceinline__ double buf_ldd(Rsrc r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
-__device__ __forceinline__ void buf_stf(Rsrc r, uint32_t voff, uint32_t soff, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, soff, 0);
-}
-__device__ __forceinline__ void buf_std(Rsrc r, uint32_t voff, uint32_t soff, double v) {
-    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
-}
 
 // T' of fine row j (wavefront-uniform) in the lane's column i
 __device__ __forceinline__ float temp_c(const StepArgsMixed& a, TCache& tc, int j, int i) {
@@ -437,11 +430,14 @@ __device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
 
 // One wavefront's work: column tile tile_x (62 output columns), band `band`
 // (rows 1 + band*band_h ...), marching down the band with a 3-row window.
-// BUF: field rows through raw buffer views (no 64-bit address arithmetic on
-// the vector ALU). The surrogate kernels and the persistent batch kernel,
-// whose scalar registers already hold the network or run state (the four
-// descriptors would add 16 and spill more), address them with plain
-// pointers: measured 3% faster there. The two address the same elements.
+// BUF: the rows the window loads go through raw buffer views (no 64-bit
+// address arithmetic on the vector ALU); the stores use plain pointers, as
+// two more descriptors would add 8 scalar registers to a kernel that
+// already spills scalars (with them: 39 spills instead of 24, 2.7% slower at
+// 4096^2). The surrogate kernels and the persistent batch kernel, whose
+// scalar registers already hold the network or run state, load through
+// plain pointers too: measured 3% faster there. Both address the same
+// elements.
 template <bool IDENT, int SURR, bool BUF = true>
 __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int tile_x, int band, unsigned& nfb,
@@ -460,7 +456,6 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
     const int Nx = a.Nx;
     const uint64_t cells = (uint64_t)Nx * a.Ny;
     const uint32_t o4 = (uint32_t)ic * 4u, o8 = (uint32_t)ic * 8u;
-    const uint32_t w4 = (uint32_t)i * 4u, w8 = (uint32_t)i * 8u;
     auto psi_row = [&](int r) {
         if constexpr (BUF)
             return buf_ldf(rsrc_of(a.psi, 4 * cells), o4, (uint32_t)(r * Nx) * 4u);
@@ -534,13 +529,8 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
 
             float* prow = a.psi_n + (size_t)j * Nx;
             double* crow = a.c_n + (size_t)j * Nx;
-            if constexpr (BUF) {
-                buf_stf(rsrc_of(a.psi_n, 4 * cells), w4, (uint32_t)(j * Nx) * 4u, psn);
-                buf_std(rsrc_of(a.c_n, 8 * cells), w8, (uint32_t)(j * Nx) * 8u, cn);
-            } else {
-                prow[i] = psn;
-                crow[i] = cn;
-            }
+            prow[i] = psn;
+            crow[i] = cn;
             if (i == 1) {
                 prow[0] = psn;
                 crow[0] = cn;
