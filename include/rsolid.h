@@ -263,7 +263,8 @@ int rs_timing(rs_ctx* ctx, int32_t which, double* avg_ms, int64_t* count);
  * check that finds a non-finite value, returning RS_ERR_NONFINITE;
  * rs_last_error names the step, the field, the number of such cells and the
  * first one in row-major order. The step's results are kept. Each check
- * reads the fields once and waits for the stream. 0 (default) turns it off. */
+ * reads the fields once and waits for the stream. 0 (default) turns it off.
+ * Runs advanced by rs_batch_step are not checked. */
 int rs_set_finite_check(rs_ctx* ctx, int32_t every);
 
 /* Measurement helper with no counterpart in the reference: launches a plain
