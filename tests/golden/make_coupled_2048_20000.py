@@ -1,4 +1,4 @@
-"""Writes configs1_2048_20000.npz: the CPU oracle's state after 20 000
+"""Writes coupled_2048_20000.npz: the CPU oracle's state after 20 000
 steps of the 2048^2 coupled-T Ti-Nb configuration (seeded planar front,
 init_config(21), FP64), reduced to what a GPU run can be compared with at
 the stated 1e-10 relative tolerance without shipping whole fields:
@@ -12,7 +12,7 @@ the stated 1e-10 relative tolerance without shipping whole fields:
 The run takes ~2 h on 8 cores (the oracle steps 2048^2 at ~11 MLUPS there);
 no pull-back falls inside it (step_pull_back = 29 936). Usage:
 
-    python tests/golden/make_configs1_2048.py [threads]
+    python tests/golden/make_coupled_2048_20000.py [threads]
 """
 import os
 import sys
@@ -80,7 +80,7 @@ def main() -> None:
         print("step", done, "elapsed %.0f s" % (time.time() - t0), flush=True)
     out = reduce_state(st)
     out.update({"N": np.int64(N), "seed": np.int64(SEED), "steps": np.int64(STEPS)})
-    path = os.path.join(HERE, "configs1_2048_20000.npz")
+    path = os.path.join(HERE, "coupled_2048_20000.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, os.path.getsize(path), "bytes")
     st.close()

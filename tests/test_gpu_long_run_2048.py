@@ -1,7 +1,7 @@
 """The 2048^2 coupled-T Ti-Nb configuration at its full length: 20 000 steps
 of the FP64 GPU path from the seeded planar front (init_config(21)) against
 the CPU oracle's state at step 20 000, held in
-tests/golden/configs1_2048_20000.npz (written by make_configs1_2048.py
+tests/golden/coupled_2048_20000.npz (written by make_coupled_2048_20000.py
 beside it; the oracle needs ~2 h on 8 cores for this run, the GPU a few
 seconds). The fixture keeps 8 full rows and a 12-column window around the
 tip column of psi and c, every 8th coarse-T row, integral figures and the
@@ -21,24 +21,24 @@ from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED, RS_P
 pytestmark = pytest.mark.gpu
 
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
-                       "configs1_2048_20000.npz")
+                       "coupled_2048_20000.npz")
 REL = 1e-10
 
 
 def say(*a):
-    print("[gpu-configs1]", *a, flush=True)
+    print("[gpu-long-run]", *a, flush=True)
     sys.stdout.flush()
 
 
 @pytest.fixture(scope="module")
 def golden():
     if not os.path.exists(FIXTURE):
-        pytest.skip("tests/golden/configs1_2048_20000.npz not generated")
+        pytest.skip("tests/golden/coupled_2048_20000.npz not generated")
     with np.load(FIXTURE, allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
 
 
-def test_configs1_20000_steps_vs_oracle(rs, golden):
+def test_fp64_path_2048_coupled_20000_steps_vs_oracle(rs, golden):
     g = golden
     N, seed, steps = int(g["N"]), int(g["seed"]), int(g["steps"])
     p = rs.config_params(N, True)
@@ -90,7 +90,7 @@ def test_configs1_20000_steps_vs_oracle(rs, golden):
     ctx.close()
 
 
-def test_configs1_fp32_path_20000_steps(rs, golden):
+def test_fp32_path_2048_coupled_20000_steps(rs, golden):
     """The FP32 path (FP32 psi/phi and fine T, FP64 c) over the same 20 000
     steps against the oracle's state: the tip advance within 1% of the
     oracle's and the solute mass within 1e-10 (the precision targets), the
@@ -133,7 +133,7 @@ def test_configs1_fp32_path_20000_steps(rs, golden):
     ctx.close()
 
 
-def test_configs1_surrogate_path_20000_steps(rs, golden):
+def test_surrogate_path_2048_coupled_20000_steps(rs, golden):
     """The surrogate path (network trained with the default options) over
     the same 20 000 steps: the tip advance within 1% of the oracle's (the
     tip-velocity target) and the fields within the surrogate step test's

@@ -108,7 +108,7 @@ def test_single_step_from_identical_state(rs, oracle_mod, case):
 @pytest.mark.parametrize("case,steps", [("cfg256_frozen", 2000), ("cfg256_coupled", 2000),
                                         ("ref512x272", 2100), ("cfg256_rotated", 2000)])
 def test_run_matches_oracle(rs, oracle_mod, case, steps):
-    """configs[0] (256^2 frozen T, 2000 steps), the same grid with the coupled
+    """The 256^2 frozen-T run (2000 steps), the same grid with the coupled
     heat equation (also with a rotated grain), and the reference parameter
     set with its step-0, 1025 and 2050 pull-backs."""
     p, kind, seed = _cases(rs)[case]
@@ -167,8 +167,8 @@ def test_pull_back_is_exact(rs, oracle_mod, off):
     ctx.close()
 
 
-def test_configs1_size_short_run(rs, oracle_mod):
-    """2048^2 coupled-T configuration (the configs[1] grid) for 60 steps."""
+def test_2048_coupled_short_run(rs, oracle_mod):
+    """The 2048^2 coupled-T configuration for 60 steps."""
     p = rs.config_params(2048, True)
     say("2048^2 coupled, 60 steps")
     st = oracle_mod.State(p)

@@ -14,7 +14,7 @@ from rsolid.abi import RS_FIELD_C, RS_FIELD_PHI, RS_FIELD_PSI, RS_FIELD_T
 
 # ---------------------------------------------------------------------------
 # set_parameters (fun.cu:1984-2058): numbers derived independently (by hand
-# from the formulas, with run.sh's flags) in the reference survey
+# from the formulas, with run.sh's flags)
 # ---------------------------------------------------------------------------
 def test_derived_reference_parameters(rs, oracle_mod):
     p = rs.reference_params(768, 832)
@@ -88,7 +88,7 @@ def test_ideal_mixing_branches(rs, oracle_mod):
 
 
 def test_common_tangent_at_reference_liquidus(rs, oracle_mod):
-    # survey pin: tangent at 2161.59 K gives c_L = 0.6317, c_S = 0.5053; the
+    # hand-derived pin: tangent at 2161.59 K gives c_L = 0.6317, c_S = 0.5053; the
     # reference's own cinf/cs (fun.cu:62) are close to it
     p = rs.reference_params(768, 832)
     st, cl, cs = oracle_mod.common_tangent(p, 2161.59, 0.63, 0.5)
