@@ -215,7 +215,7 @@ StepArgsMixed step_args_mixed(rs_ctx* ctx) {
     a.c = ctx->c[ctx->cur];
     a.psi_n = ctx->psi32[ctx->cur ^ 1];
     a.c_n = ctx->c[ctx->cur ^ 1];
-    a.lh = ctx->lh;
+    a.lh = reinterpret_cast<float*>(ctx->lh);  // FP32 sums in the FP64-sized buffer
     a.T32 = ctx->T32[ctx->tcur];
     a.Nx = p.Nx;
     a.Ny = p.Ny;
@@ -292,7 +292,9 @@ TempArgs temp_args(rs_ctx* ctx, int64_t step) {
     const rs_constants& k = ctx->k;
     const double tau0 = tau0_of(p);
     TempArgs t;
-    t.lh = ctx->lh;
+    // the FP32 paths keep their latent-heat column sums in FP32
+    t.lh = ctx->mixed ? nullptr : ctx->lh;
+    t.lh32 = ctx->mixed ? reinterpret_cast<const float*>(ctx->lh) : nullptr;
     t.Nx = p.Nx;
     t.NxT = k.NxT;
     t.NyT = k.NyT;

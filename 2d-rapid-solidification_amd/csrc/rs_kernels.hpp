@@ -58,7 +58,8 @@ struct StepArgsF64 {
 struct TempArgs {
     const double* T;
     double* Tn;
-    const double* lh;
+    const double* lh;     // latent-heat column sums (FP64 path), or null
+    const float* lh32;    // the same in FP32 (FP32 paths), or null
     int Nx, NxT, NyT, NxT_all, ratio, ile, ratio_dt_dtT;
     double vadv;      // Vp_ramp(t) tau0 / (S W0)
     double dxT, dtT, DT, Gmu /* Gamma*muk0 */, Lcp, Tcold, Thot;
@@ -70,6 +71,24 @@ struct TempArgs {
 // (fun.cu:1537-1578) for coarse cell (iT, jT) < (NxT, NyT), race-free
 // boundaries: the owner of row NyT-2 writes row 0, the owner of row 1 writes
 // row NyT-1, Dirichlet columns by their own threads
+// sum of one row's latent-heat column sums i0..in, in column order, in FP64
+template <typename L>
+__device__ __forceinline__ double lh_block_sum(const L* row, int i0, int in) {
+    double integral = 0.0;
+    if (in - i0 == 9) {
+        // a full block of 10 columns (the library's ratio): all loads in
+        // flight before the in-order sum
+        double v[10];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) v[q] = (double)row[i0 + q];
+#pragma unroll
+        for (int q = 0; q < 10; ++q) integral += v[q];
+    } else {
+        for (int i = i0; i <= in; ++i) integral += (double)row[i];
+    }
+    return integral;
+}
+
 __device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int jT) {
     const int64_t P = a.NxT_all;
     if (iT == 0 || iT == a.NxT - 1) {
@@ -94,19 +113,9 @@ __device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int 
         int in = i0 + r - 1;
         if (i0 < 1) i0 = 1;
         if (in > a.Nx - 2) in = a.Nx - 2;
-        const double* row = a.lh + (int64_t)(jT - 1) * a.Nx;
-        double integral = 0.0;
-        if (in - i0 == 9) {
-            // a full block of 10 columns (the library's ratio): all loads in
-            // flight before the in-order sum
-            double v[10];
-#pragma unroll
-            for (int q = 0; q < 10; ++q) v[q] = row[i0 + q];
-#pragma unroll
-            for (int q = 0; q < 10; ++q) integral += v[q];
-        } else {
-            for (int i = i0; i <= in; ++i) integral += row[i];
-        }
+        const int64_t off = (int64_t)(jT - 1) * a.Nx;
+        const double integral =
+            a.lh32 ? lh_block_sum(a.lh32 + off, i0, in) : lh_block_sum(a.lh + off, i0, in);
         dphidt = integral * 0.5;
     }
     const double lap = (T10 + Tm0 + T01 + T0m - 4.0 * T00) / (a.dxT * a.dxT);

@@ -555,7 +555,7 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
             if (a.lh_on && j <= a.lh_last_row) {
                 lh_acc += psn - Z.q.psi;
                 if (j % 10 == 0) {
-                    a.lh[(size_t)(j / 10 - 1) * Nx + i] = (double)lh_acc;
+                    a.lh[(size_t)(j / 10 - 1) * Nx + i] = lh_acc;
                     lh_acc = 0.0f;
                 }
             }

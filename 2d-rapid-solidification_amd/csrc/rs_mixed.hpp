@@ -18,7 +18,7 @@ struct StepArgsMixed {
     const double* c;
     float* psi_n;
     double* c_n;
-    double* lh;
+    float* lh;  // latent-heat column sums (FP32: the step accumulates them in FP32)
     const float* T32;        // coarse T - kMixedTref in FP32 (NxT_all pitch)
     int Nx, Ny, band_h, nbands, lh_last_row, lh_on;
     int tfc, NxT_all, ile;
