@@ -151,7 +151,9 @@ __device__ __forceinline__ void antitrap_band(const T* psi, const T* psi_n, cons
     double Jy_prev = (double)at_yface(k, M, Z);
     for (int j = jb; j <= je; ++j) {
         const AtRow<R> P = at_row<R>(nx);
-        if (j + 2 <= je + 1) nx = at_load(psi, psi_n, c, c_n, Nx, j + 2, ic);
+        // unconditional (the last row reloads row je+1, whose values are not
+        // used), so the compiler need not wait on the stores before it
+        nx = at_load(psi, psi_n, c, c_n, Nx, min(j + 2, je + 1), ic);
         const double Jx = (double)at_flux(k, Z.f, Z.fr, Z.g, Z.gr, Z.c, Z.cr, P.f, P.fr, M.f, M.fr);
         const double Jxm = at_up(Jx);
         const double Jy = (double)at_yface(k, Z, P);

@@ -95,9 +95,10 @@ __global__ __launch_bounds__(256) void step_fp64_kernel(StepArgsF64 a) {
 
     for (int j = jb; j <= je; ++j) {
         const double cur_psi = nx_psi, cur_c = nx_c;
-        if (j + 2 <= je + 1) {
-            nx_psi = a.psi[(int64_t)(j + 2) * Nx + ic];
-            nx_c = a.c[(int64_t)(j + 2) * Nx + ic];
+        {  // unconditional, the last row reloading row je+1 (no wait on the stores)
+            const int jn = min(j + 2, je + 1);
+            nx_psi = a.psi[(int64_t)jn * Nx + ic];
+            nx_c = a.c[(int64_t)jn * Nx + ic];
         }
         const Cell p = make_cell(a, cur_psi, cur_c, ic, j + 1);
         const double phL_p = shfl_up1(p.phi), phR_p = shfl_dn1(p.phi);
