@@ -90,21 +90,36 @@ constexpr int kRatio = 10;
 // values are fixed for a lane within one coarse row (10 fine rows), so A and
 // B are cached and each fine row costs one FMA; the operations, hence the
 // values, are those of the direct form.
+// The coarse T' values a workgroup's wavefronts interpolate from, staged in
+// LDS at the start of the single-run kernel (p null: read from the FP32
+// coarse grid in global memory, as the batch kernel does). Reading them from
+// LDS keeps the once-per-coarse-row loads off the vector-memory counter,
+// which on gfx950 also counts the row's stores (surrogate kernels: 2%).
+typedef const __attribute__((address_space(3))) float* LdsF;
+struct TPatch {
+    LdsF p;           // the patch in LDS (valid when on)
+    int on;
+    int iT0, jT0, W;  // first coarse column and row held, row pitch
+};
+constexpr int kTPatchN = 1024;  // floats: 4 bands of up to 200 rows x 9 columns
+
 struct TCache {
     int jT;        // coarse row of the cached terms (-1: none)
     float A, B;
     float x;
-    uint32_t off;  // byte offset of the lane's coarse column iT in a row
+    uint32_t off;  // byte offset of the lane's coarse column iT in a row (of the patch)
+    TPatch tp;
 };
 
-__device__ __forceinline__ TCache tcache_init(const StepArgsMixed& a, int i) {
+__device__ __forceinline__ TCache tcache_init(const StepArgsMixed& a, int i, const TPatch& tp) {
     TCache tc;
     tc.jT = -1;
     tc.A = tc.B = 0.f;
     const int g = i - 1 + a.ile;
     const int iT = g / kRatio;
     tc.x = (float)(g - kRatio * iT) * 0.1f;
-    tc.off = (uint32_t)iT * 4u;
+    tc.off = (uint32_t)(tp.on ? iT - tp.iT0 : iT) * 4u;
+    tc.tp = tp;
     return tc;
 }
 
@@ -138,10 +153,21 @@ __device__ __forceinline__ float temp_c(const StepArgsMixed& a, TCache& tc, int 
         const int jT = g / kRatio;
         if (jT != tc.jT) {
             tc.jT = jT;
-            const float* r0 = a.T32 + (size_t)jT * a.NxT_all;
-            const float* r1 = r0 + a.NxT_all;
-            const float f00 = ld_at(r0, tc.off), f10 = ld_at(r0, tc.off + 4u);
-            const float f01 = ld_at(r1, tc.off), f11 = ld_at(r1, tc.off + 4u);
+            float f00, f10, f01, f11;
+            if (tc.tp.on) {
+                const int q = (jT - tc.tp.jT0) * tc.tp.W + (int)(tc.off >> 2);
+                f00 = tc.tp.p[q];
+                f10 = tc.tp.p[q + 1];
+                f01 = tc.tp.p[q + tc.tp.W];
+                f11 = tc.tp.p[q + tc.tp.W + 1];
+            } else {
+                const float* r0 = a.T32 + (size_t)jT * a.NxT_all;
+                const float* r1 = r0 + a.NxT_all;
+                f00 = ld_at(r0, tc.off);
+                f10 = ld_at(r0, tc.off + 4u);
+                f01 = ld_at(r1, tc.off);
+                f11 = ld_at(r1, tc.off + 4u);
+            }
             const float a10 = f10 - f00, a01 = f01 - f00, a11 = (f11 - f10) - (f01 - f00);
             tc.A = fmaf(a11, tc.x, a01);
             tc.B = fmaf(a10, tc.x, f00);
@@ -441,7 +467,7 @@ __device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
 template <bool IDENT, int SURR, bool BUF = true>
 __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int tile_x, int band, unsigned& nfb,
-                                          unsigned& nall) {
+                                          unsigned& nall, const TPatch& tp) {
     const int lane = threadIdx.x;
     // wavefront-uniform (blockDim.x is the wavefront): row addresses then
     // live in scalar registers and each access adds a 32-bit lane offset
@@ -470,7 +496,7 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
     };
 
     Slot s0, s1, s2;
-    TCache tc = tcache_init(a, ic);
+    TCache tc = tcache_init(a, ic, tp);
     const bool real = i < a.Nx;  // lanes past the last column duplicate it
     fill_slot<SURR>(a, lntab, sl, tc, jb - 1, psi_row(jb - 1), c_row(jb - 1), ic, real, s0, nfb,
                     nall);
@@ -579,6 +605,37 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
     }
 }
 
+// Loads into buf the coarse T' patch the wavefronts of a workgroup
+// interpolate from: column tile `tile`, bands band0 .. band0 + nb - 1, rows
+// jb - 1 .. je + 1 of each (the coarse rows and columns temp_c reaches for
+// them, plus the next row and column). Not loaded (on = 0, temp_c reads the
+// global grid) without coupled T or when it exceeds kTPatchN floats.
+__device__ __forceinline__ TPatch tpatch_load(const StepArgsMixed& a, float* buf, int tile,
+                                              int band0, int nb, int tid, int nthr) {
+    TPatch t{(LdsF)buf, 0, 0, 0, 0};
+    if (!a.tfc || band0 >= a.nbands) return t;
+    const int b1 = min(band0 + nb, a.nbands) - 1;
+    const int r_lo = band0 * a.band_h;                                    // jb - 1, first band
+    const int r_hi = min(1 + (b1 + 1) * a.band_h - 1, a.Ny - 2) + 1;      // je + 1, last band
+    const int jT0 = (r_lo - 1 + a.dxTi) / kRatio;
+    const int jT1 = (r_hi - 1 + a.dxTi) / kRatio + 1;
+    const int i_lo = tile * kTileOut;
+    const int i_hi = min(tile * kTileOut + kWave - 1, a.Nx - 1);
+    const int iT0 = (i_lo - 1 + a.ile) / kRatio;
+    const int iT1 = (i_hi - 1 + a.ile) / kRatio + 1;
+    const int W = iT1 - iT0 + 1, H = jT1 - jT0 + 1;
+    if (W * H > kTPatchN) return t;
+    for (int q = tid; q < W * H; q += nthr) {
+        const int y = q / W;
+        buf[q] = a.T32[(size_t)(jT0 + y) * a.NxT_all + iT0 + (q - y * W)];
+    }
+    t.on = 1;
+    t.iT0 = iT0;
+    t.jT0 = jT0;
+    t.W = W;
+    return t;
+}
+
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
 // SURR - 1 (kActBf16, kActSplit, kActFp16, kActFp16Pk). 4 wavefronts per SIMD (<= 128
 // VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
@@ -587,15 +644,21 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
 template <bool IDENT, int SURR>
 __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
     __shared__ StepLds<SURR> L;
-    load_step_lds<SURR>(L, a.mlp, a.trust, threadIdx.y * blockDim.x + threadIdx.x,
-                        blockDim.x * blockDim.y);
-    __syncthreads();
-    unsigned nfb = 0, nall = 0;
+    // the coarse T' patch pays in the surrogate kernels (2%); the exact-table
+    // kernel, whose row is short, measured no gain and reads the global grid
+    __shared__ float tpatch[SURR != 0 ? kTPatchN : 1];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, nthr = blockDim.x * blockDim.y;
+    load_step_lds<SURR>(L, a.mlp, a.trust, tid, nthr);
     // logical workgroup (column tile, band group) in XCD order
     const int ntx = gridDim.x;
     const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
+    TPatch tp{};
+    if constexpr (SURR != 0)
+        tp = tpatch_load(a, tpatch, g % ntx, (g / ntx) * blockDim.y, blockDim.y, tid, nthr);
+    __syncthreads();
+    unsigned nfb = 0, nall = 0;
     step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
-                                      (g / ntx) * blockDim.y + threadIdx.y, nfb, nall);
+                                      (g / ntx) * blockDim.y + threadIdx.y, nfb, nall, tp);
     if constexpr (SURR != 0) {
         if (threadIdx.x == 0 && nall > 0) {
             atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
@@ -689,7 +752,7 @@ __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaun
         const int bg = local / R.ntx;
         unsigned nfb = 0, nall = 0;
         step_tile<IDENT, SURR, false>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
-                               nfb, nall);
+                               nfb, nall, TPatch{});
         if constexpr (SURR != 0) {
             if (threadIdx.x == 0 && nall > 0) {
                 atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
