@@ -394,11 +394,15 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     ctx->nT = (int64_t)ctx->k.NxT_all * ctx->k.NyT_all;
     if (const char* e = getenv("RS_BAND_H")) {
         const int v = atoi(e);
-        if (v >= 10 && v % 10 == 0) ctx->band_h = v;
+        if (v >= 10 && v <= 2000 && v % 10 == 0) ctx->band_h = v;
     }
     if (const char* e = getenv("RS_WPB")) {
         const int v = atoi(e);
         if (v >= 1 && v <= 4) ctx->wpb = v;
+    }
+    if (const char* e = getenv("RS_SURR_ALWAYS")) {
+        const int v = atoi(e);
+        if (v == 0 || v == 1) ctx->surr_always = v;
     }
     hipError_t e;
     if ((e = hipGetDevice(&ctx->device)) != hipSuccess ||
@@ -643,6 +647,11 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             if (ctx->surrogate) {
                 if (!surr_ready(ctx))
                     return fail(ctx, RS_ERR_STATE, "surrogate path needs trained weights");
+                // after the first 64 steps and then every 1024
+                if (ctx->surr_always < 0 && ctx->surr_steps++ % 1024 == 64) {
+                    const int rc = surr_update_policy(ctx);
+                    if (rc) return rc;
+                }
                 surr_step_args(ctx, a);
             }
             const double* o = ctx->orient;

@@ -36,6 +36,7 @@ struct SurrState;
 void surr_free(rs_ctx* ctx);
 bool surr_ready(const rs_ctx* ctx);
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a);
+int surr_update_policy(rs_ctx* ctx);
 
 }  // namespace rs
 
@@ -68,6 +69,10 @@ struct rs_ctx {
     int* tiprow = nullptr;
     int cur = 0, tcur = 0;
     int band_h = 40, wpb = 4;
+    // surrogate steps: -1 choose per run whether every row evaluates the
+    // network (surr_update_policy), 0 never, 1 always (RS_SURR_ALWAYS)
+    int surr_always = -1;
+    int64_t surr_steps = 0;  // surrogate steps taken (policy schedule)
     std::string err;
     // HIP-event timing of the fused step kernel [0] and the coarse T kernel
     // [1] on every timing-th step (0: off)

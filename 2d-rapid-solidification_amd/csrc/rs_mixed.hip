@@ -268,7 +268,7 @@ struct SurrLds {
     const uint64_t* trust;
 };
 
-template <int FMT>
+template <int FMT, bool ALWAYS = false>
 __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* lntab,
                                          const SurrLds& sl, float Tp, double c, float h, int i,
                                          int j, bool real, float& dG, double& mu, unsigned& nfb,
@@ -290,15 +290,18 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
         const int ib = min((int)((bin + 1.0f) * (0.5f * kTrustN)), kTrustN - 1);
         ok = (sl.trust[ib] >> ia) & 1ULL;
     }
+    // a row with no trusted cell takes the exact routine in every cell, and
     // the network runs only if some cell of the row uses it (wavefront-
-    // uniform): a row wholly outside the trusted domain takes the exact
-    // routine alone, with the same values
+    // uniform). ALWAYS drops that test, for runs where such rows are rare:
+    // the branch around the network costs ~4% of the step.
     const bool any = __ballot(ok && real) != 0;
     ok = any && (ok || !real);
-    nfb += (unsigned)__popcll(__ballot(real && !ok));
-    nall += (unsigned)__popcll(__ballot(real));
+    // counts (wavefront-uniform): fallback cells and, from bit 20, rows
+    // without the network; all cells and, from bit 20, rows
+    nfb += (unsigned)__popcll(__ballot(real && !ok)) + (any ? 0u : (1u << 20));
+    nall += (unsigned)__popcll(__ballot(real)) + (1u << 20);
     mlp::f4 y = {0.f, 0.f, 0.f, 0.f};
-    if (any) y = mlp::forward<FMT>(F, mlp::input_frag(ain, bin), lane);
+    if (ALWAYS || any) y = mlp::forward<FMT>(F, mlp::input_frag(ain, bin), lane);
     if (ok) {
         const double onemc = 1.0 - c;
         double lr = 0.0;
@@ -310,6 +313,15 @@ __device__ __forceinline__ void thermo_s(const StepArgsMixed& a, const double* l
     } else {
         thermo_m(a, lntab, Tp, c, h, dG, mu);
     }
+}
+
+// adds a wavefront's packed counts (cells in bits 0-19, rows from bit 20)
+// to one counter slot
+__device__ __forceinline__ void add_counts(unsigned long long* q, unsigned nfb, unsigned nall) {
+    atomicAdd(&q[0], (unsigned long long)(nfb & 0xFFFFFu));
+    atomicAdd(&q[1], (unsigned long long)(nall & 0xFFFFFu));
+    atomicAdd(&q[2], (unsigned long long)(nfb >> 20));
+    atomicAdd(&q[3], (unsigned long long)(nall >> 20));
 }
 
 struct CellM {
@@ -330,7 +342,8 @@ __device__ __forceinline__ CellM make_cell_m(const StepArgsMixed& a, const doubl
     q.S = qq<float>(a.AA, a.q_solid, psi) * (cf * (1.0f - cf));
     const float h = 0.5f * (1.0f + gg<float>(psi));
     if constexpr (SURR != 0)
-        thermo_s<SURR - 1>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb, nall);
+        thermo_s<(SURR - 1) & 3, (SURR > 4)>(a, lntab, sl, Tp, c, h, i, j, real, q.dG, q.mu, nfb,
+                                            nall);
     else
         thermo_m(a, lntab, Tp, c, h, q.dG, q.mu);
     return q;
@@ -433,6 +446,8 @@ template <>
 struct StepLds<3> : StepLds<1> {};
 template <>
 struct StepLds<4> : StepLds<1> {};
+template <>
+struct StepLds<7> : StepLds<1> {};
 
 template <int SURR>
 __device__ __forceinline__ void load_step_lds(StepLds<SURR>& L, const MlpInfer* mlp,
@@ -637,7 +652,8 @@ __device__ __forceinline__ TPatch tpatch_load(const StepArgsMixed& a, float* buf
 }
 
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
-// SURR - 1 (kActBf16, kActSplit, kActFp16, kActFp16Pk). 4 wavefronts per SIMD (<= 128
+// (SURR - 1) & 3 (kActBf16, kActSplit, kActFp16, kActFp16Pk); 7: FP16 with the
+// network evaluated in every row. 4 wavefronts per SIMD (<= 128
 // VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
 // 4096^2 step takes 0.191 ms instead of 0.166 ms. The surrogate variants
 // hold the MFMA operands and run at 2 (3 spills 52 registers).
@@ -660,10 +676,8 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
     step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
                                       (g / ntx) * blockDim.y + threadIdx.y, nfb, nall, tp);
     if constexpr (SURR != 0) {
-        if (threadIdx.x == 0 && nall > 0) {
-            atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
-            atomicAdd(&a.fb_count[1], (unsigned long long)nall);
-        }
+        if (threadIdx.x == 0 && nall > 0)
+            add_counts(a.fb_count + (size_t)(g % kCountSlots) * kCountStride, nfb, nall);
     }
 }
 
@@ -677,6 +691,11 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
             hipLaunchKernelGGL((step_mixed_kernel<true, 4>), grid, block, 0, s, a);
         else
             hipLaunchKernelGGL((step_mixed_kernel<false, 4>), grid, block, 0, s, a);
+    } else if (surr && a.s_fmt == kActFp16 && a.s_always) {
+        if (ident)
+            hipLaunchKernelGGL((step_mixed_kernel<true, 7>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((step_mixed_kernel<false, 7>), grid, block, 0, s, a);
     } else if (surr && a.s_fmt == kActFp16) {
         if (ident)
             hipLaunchKernelGGL((step_mixed_kernel<true, 3>), grid, block, 0, s, a);
@@ -754,10 +773,8 @@ __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaun
         step_tile<IDENT, SURR, false>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
                                nfb, nall, TPatch{});
         if constexpr (SURR != 0) {
-            if (threadIdx.x == 0 && nall > 0) {
-                atomicAdd(&a.fb_count[0], (unsigned long long)nfb);
-                atomicAdd(&a.fb_count[1], (unsigned long long)nall);
-            }
+            if (threadIdx.x == 0 && nall > 0)
+                add_counts(a.fb_count + (size_t)(item % kCountSlots) * kCountStride, nfb, nall);
         }
     }
 }

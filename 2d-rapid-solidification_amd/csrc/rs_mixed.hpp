@@ -13,6 +13,10 @@ namespace rs {
 // by the mixed path and its SGTE polynomial table (rs_sgte_poly.inc).
 constexpr double kMixedTref = 1941.0;
 
+constexpr int kCountSlots = 64;
+constexpr int kCountStride = 16;
+constexpr size_t kCountBytes = sizeof(unsigned long long) * kCountSlots * kCountStride;
+
 struct StepArgsMixed {
     const float* psi;
     const double* c;
@@ -38,10 +42,15 @@ struct StepArgsMixed {
     // input/output normalisation
     const MlpInfer* mlp;
     const uint64_t* trust;
+    // surrogate counts, kCountSlots slots of kCountStride words (one
+    // 128-byte line each; a workgroup adds to slot id % kCountSlots, so the
+    // atomics of a launch spread over 64 lines instead of queueing on one):
+    // [0] fallback cells, [1] all cells, [2] rows without the network, [3] rows
     unsigned long long* fb_count;
     float s_a_scale, s_a_off;  // a = T' s_a_scale + s_a_off
     float y_mean[4], y_std[4];
     int s_fmt;                 // activation format (kActBf16 ... kActFp16Pk)
+    int s_always;              // FP16 format: evaluate the network in every row (no skip test)
 };
 
 // ident: the crystal orientation is the identity (the reference's only

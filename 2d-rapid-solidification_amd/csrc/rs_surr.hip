@@ -22,7 +22,11 @@ struct SurrState {
     float* v = nullptr;
     MlpFrags* F = nullptr;
     uint64_t* trust = nullptr;               // device copy, kTrustN words
-    unsigned long long* count = nullptr;     // fallback, all evaluations
+    // kCountSlots slots of [0] fallback cells, [1] all cells, [2] rows
+    // without the network, [3] all rows (one row = one wavefront's cells)
+    unsigned long long* count = nullptr;
+    unsigned long long seen_rows = 0, seen_skip = 0;  // [3], [2] at the last policy update
+    bool always = false;  // evaluate the network in every row
     MlpNorm norm{};
     double T_lo = 0.0, T_hi = 0.0, tol = 1e-4;
     int fmt = kActFp16;  // inference activation format (training: BF16, split unless kActBf16)
@@ -78,8 +82,8 @@ int ensure_state(rs_ctx* ctx) {
     HIPCHK(ctx, hipMalloc(&s->v, sizeof(float) * kNParam));
     HIPCHK(ctx, hipMalloc(&s->F, sizeof(MlpFrags)));
     HIPCHK(ctx, hipMalloc(&s->trust, sizeof(uint64_t) * kTrustN));
-    HIPCHK(ctx, hipMalloc(&s->count, sizeof(unsigned long long) * 2));
-    HIPCHK(ctx, hipMemsetAsync(s->count, 0, sizeof(unsigned long long) * 2, ctx->stream));
+    HIPCHK(ctx, hipMalloc(&s->count, kCountBytes));
+    HIPCHK(ctx, hipMemsetAsync(s->count, 0, kCountBytes, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(s->trust, 0, sizeof(uint64_t) * kTrustN, ctx->stream));
     return RS_OK;
 }
@@ -198,6 +202,24 @@ int mse_of(rs_ctx* ctx, const float* dX, const float* dYn, int64_t n, float* dO,
     return RS_OK;
 }
 
+// the four counts summed over the slots
+void sum_counts(const unsigned long long* slots, unsigned long long c[4]) {
+    for (int k = 0; k < 4; ++k) c[k] = 0;
+    for (int q = 0; q < kCountSlots; ++q)
+        for (int k = 0; k < 4; ++k) c[k] += slots[(size_t)q * kCountStride + k];
+}
+
+// the row counts skip (rows without the network) and rows
+// since the counters were last cleared; see surr_update_policy
+void surr_policy_apply(SurrState* s, unsigned long long skip_all, unsigned long long rows_all) {
+    if (rows_all < s->seen_rows || skip_all < s->seen_skip) s->seen_rows = s->seen_skip = 0;
+    const unsigned long long rows = rows_all - s->seen_rows, skip = skip_all - s->seen_skip;
+    if (rows < 4096) return;
+    s->always = skip * 50 < rows;
+    s->seen_rows = rows_all;
+    s->seen_skip = skip_all;
+}
+
 }  // namespace
 
 namespace rs {
@@ -227,6 +249,27 @@ void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
         a.y_std[k] = s->norm.y_std[k];
     }
     a.s_fmt = s->fmt;
+    a.s_always = ctx->surr_always >= 0 ? ctx->surr_always : s->always;
+}
+
+// Rows with no trusted cell skip the network behind a branch that costs ~4%
+// of a step whenever such rows are rare (a front grown inside the sampled
+// (T, c) box); once the rows counted since the last update number at least
+// 4096 and fewer than 2% of them skipped the network, the steps evaluate it
+// in every row (same results: such rows discard it), and go back to the
+// branch when the share rises again. Waits for the stream (it reads the
+// counts), so rs_step calls it once per 1024 surrogate steps.
+int surr_update_policy(rs_ctx* ctx) {
+    SurrState* s = ctx->surr;
+    if (!s || !s->ready) return RS_OK;
+    std::vector<unsigned long long> slots(kCountBytes / sizeof(unsigned long long));
+    HIPCHK(ctx, hipMemcpyAsync(slots.data(), s->count, kCountBytes, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c[4];
+    sum_counts(slots.data(), c);
+    surr_policy_apply(s, c[2], c[3]);
+    return RS_OK;
 }
 
 }  // namespace rs
@@ -591,15 +634,21 @@ int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out) {
 int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset) {
     if (!ctx || !out) return RS_ERR_ARG;
     if (!ctx->surr) return fail(ctx, RS_ERR_STATE, "no surrogate");
-    unsigned long long c[2];
-    HIPCHK(ctx, hipMemcpyAsync(c, ctx->surr->count, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> slots(kCountBytes / sizeof(unsigned long long));
+    HIPCHK(ctx, hipMemcpyAsync(slots.data(), ctx->surr->count, kCountBytes, hipMemcpyDeviceToHost,
+                               ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c[4];
+    sum_counts(slots.data(), c);
     out[0] = (double)c[0];
     out[1] = (double)c[1];
     out[2] = c[1] ? (double)c[0] / (double)c[1] : 0.0;
+    out[3] = (double)c[2];
+    out[4] = (double)c[3];
     if (reset) {
-        HIPCHK(ctx, hipMemsetAsync(ctx->surr->count, 0, sizeof c, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->surr->count, 0, kCountBytes, ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->surr->seen_rows = ctx->surr->seen_skip = 0;
     }
     return RS_OK;
 }

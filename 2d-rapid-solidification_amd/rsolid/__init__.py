@@ -351,9 +351,10 @@ class Context:
         return out
 
     def surrogate_stats(self, reset: bool = False) -> dict:
-        v = (C.c_double * 3)()
+        v = (C.c_double * 5)()
         self._chk(lib.rs_surrogate_stats(self._h, v, 1 if reset else 0), "rs_surrogate_stats")
-        return {"fallback": v[0], "evaluated": v[1], "fallback_frac": v[2]}
+        return {"fallback": v[0], "evaluated": v[1], "fallback_frac": v[2],
+                "rows_without_network": v[3], "rows": v[4]}
 
     # -- formats -------------------------------------------------------------
     def download_columns(self, field: int, i0: int, ncols: int) -> np.ndarray:

@@ -418,7 +418,10 @@ def main() -> int:
             out["tip_velocity_err"] = out["paths"][a.path]["tip_velocity_err"]
     if "surrogate" in res:
         r = res["surrogate"]
-        out["surrogate"] = {"fallback_frac": r["fallback"]["fallback_frac"],
+        fbs = r["fallback"]
+        out["surrogate"] = {"fallback_frac": fbs["fallback_frac"],
+                            "rows_without_network_frac":
+                                fbs["rows_without_network"] / max(1.0, fbs["rows"]),
                             "train_s": round(r["train"]["train_s"], 3),
                             "train_steps": int(r["train"]["steps"]),
                             "trusted_bin_frac": r["train"]["trusted_frac"],

@@ -387,9 +387,11 @@ int rs_surrogate_grad(rs_ctx* ctx, const float* X, const float* Yn, int32_t n, f
  * mu_L, mu_S (J/mol, ideal terms added exactly) and 1.0 where the point is
  * in the domain and its bin trusted, else 0.0. */
 int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out);
-/* Fallback counts of the fused kernel since the last reset: out[0] cell
- * evaluations by the exact routine, out[1] all evaluations, out[2] their
- * ratio. reset != 0 clears the counters afterwards. */
+/* Counts of the fused kernel since the last reset (out: 5 values): out[0]
+ * cell evaluations by the exact routine, out[1] all evaluations, out[2]
+ * their ratio, out[3] wavefront rows (62 cells) with no trusted cell, which
+ * need not evaluate the network, out[4] all wavefront rows. reset != 0
+ * clears the counters afterwards. */
 int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset);
 
 /* ---- batched ensemble ----------------------------------------------------

@@ -222,3 +222,29 @@ def test_surrogate_conserves_solute_exactly(rs):
     assert st["fallback_frac"] < 0.5
     assert abs(drift) < 1e-13
     ctx.close()
+
+
+def test_network_in_every_row_gives_the_same_fields(rs, trained, monkeypatch):
+    """Rows with no cell in the network's domain skip it behind a branch;
+    evaluating it in every row instead (RS_SURR_ALWAYS=1, or the per-run
+    choice once such rows are rare) discards it there, so psi and c stay
+    bitwise equal to the branched steps."""
+    ctx_t, _ = trained
+    P, meta = ctx_t.surrogate_get()
+    ps = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
+    out = {}
+    for mode in ("0", "1", None):
+        if mode is None:
+            monkeypatch.delenv("RS_SURR_ALWAYS", raising=False)
+        else:
+            monkeypatch.setenv("RS_SURR_ALWAYS", mode)
+        say("network in every row:", mode)
+        ctx = rs.Context(ps)
+        ctx.init_config(7)
+        ctx.surrogate_set(P, meta)
+        ctx.step(800)
+        out[mode] = (ctx.download(RS_FIELD_PSI), ctx.download(RS_FIELD_C))
+        ctx.close()
+    for mode in ("1", None):
+        assert np.array_equal(out[mode][0], out["0"][0])
+        assert np.array_equal(out[mode][1], out["0"][1])
