@@ -237,7 +237,10 @@ int rs_set_constants(rs_ctx* ctx, const rs_constants* k);
  * context's current step: pull-back when step % step_pull_back == 0, the
  * fused phi/psi/c update, latent heat, coarse T sub-steps, swaps and the
  * l_offset_Vp update. Work is enqueued on the context stream; the call
- * returns after the last pull-back decision (which needs the tip column). */
+ * returns after the last pull-back decision (which needs the tip column).
+ * Surrogate contexts also wait for the stream once after their first 64
+ * surrogate steps and then once per 1024, to read the row counts that pick
+ * the step kernel variant (unless RS_SURR_ALWAYS fixed it at rs_create). */
 int rs_step(rs_ctx* ctx, int64_t nsteps);
 
 /* Pieces of the loop body, for drivers that interleave their own output. */
