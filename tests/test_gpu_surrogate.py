@@ -244,6 +244,14 @@ def test_network_in_every_row_gives_the_same_fields(rs, trained, monkeypatch):
         ctx.surrogate_set(P, meta)
         ctx.step(800)
         out[mode] = (ctx.download(RS_FIELD_PSI), ctx.download(RS_FIELD_C))
+        st = ctx.surrogate_stats()
+        say("stats", st)
+        # a wavefront row evaluates 1-64 cells (its 62 columns and the two
+        # halo columns, cut at the grid's edge), and every cell of a row
+        # without a trusted cell takes the exact routine
+        assert st["rows"] > 0 and 0 <= st["rows_without_network"] <= st["rows"]
+        assert st["rows"] <= st["evaluated"] <= 64 * st["rows"]
+        assert st["fallback"] >= st["rows_without_network"]
         ctx.close()
     for mode in ("1", None):
         assert np.array_equal(out[mode][0], out["0"][0])
