@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "rs_host.hpp"
+#include "rs_mlp.hpp"
 
 using namespace rs;
 
@@ -25,6 +26,9 @@ struct rs_batch {
     BatchDyn* d_dyn = nullptr;
     int* d_item0 = nullptr;
     int* d_cell0 = nullptr;
+    // surrogate batches: the AND of the runs' trust bitmaps (a cell takes
+    // the network only where every run's own residual bound holds)
+    uint64_t* d_trust = nullptr;
     std::vector<BatchRun> h_runs;
     std::vector<BatchDyn> h_dyn;
     std::vector<int> band_h;     // per run (autotuned)
@@ -72,6 +76,25 @@ int build_tables(rs_batch* b) {
     std::vector<int> item0(n + 1, 0), cell0(n + 1, 0);
     bool ident = true, at_any = false;
     int nsub = 0;
+    // the persistent kernel stages one network in LDS for every run: the
+    // runs must share it, and each keeps its own trust bound
+    const bool surr = b->runs[0]->surrogate && surr_ready(b->runs[0]);
+    if (surr) {
+        uint64_t tr[kTrustN];
+        memcpy(tr, surr_trust_host(b->runs[0]), sizeof tr);
+        for (int r = 1; r < n; ++r) {
+            rs_ctx* c = b->runs[r];
+            if (!surr_ready(c) || surr_identity(c) != surr_identity(b->runs[0]))
+                return bfail(b, RS_ERR_STATE,
+                             "the runs of a surrogate batch must share one network "
+                             "(rs_batch_surrogate_train installs it in every run)");
+            const uint64_t* t = surr_trust_host(c);
+            for (int q = 0; q < kTrustN; ++q) tr[q] &= t[q];
+        }
+        if (!b->d_trust) BCHK(b, hipMalloc(&b->d_trust, sizeof tr));
+        BCHK(b, hipMemcpyAsync(b->d_trust, tr, sizeof tr, hipMemcpyHostToDevice, b->stream));
+        BCHK(b, hipStreamSynchronize(b->stream));
+    }
     for (int r = 0; r < n; ++r) {
         rs_ctx* c = b->runs[r];
         c->band_h = b->band_h[r];
@@ -79,7 +102,10 @@ int build_tables(rs_batch* b) {
         const rs_constants& k = c->k;
         BatchRun& R = b->h_runs[r];
         R.a = mixed_step_args(c);
-        if (c->surrogate) surr_step_args(c, R.a);
+        if (c->surrogate) {
+            surr_step_args(c, R.a);
+            if (surr) R.a.trust = b->d_trust;
+        }
         R.a.band_h = b->band_h[r];
         R.a.nbands = (p.Ny - 2 + R.a.band_h - 1) / R.a.band_h;
         for (int q = 0; q < 2; ++q) {
@@ -232,6 +258,9 @@ int do_steps(rs_batch* b, int64_t nsteps) {
                 int rc = rs_tip_column(c, &it);
                 if (!rc) rc = rs_pull_back(c, it - c->p.i_tip_target);
                 if (rc) return bfail(b, rc, c->err.c_str());
+                // the shift runs on the run's own stream; the batch launches
+                // below (another stream) read the shifted fields
+                BCHK(b, hipStreamSynchronize(c->stream));
             }
         }
         const int64_t seg = std::min(nsteps - done, steps_to_pull_back(b));
@@ -369,7 +398,7 @@ int rs_batch_destroy(rs_batch* b) {
     if (!b) return RS_ERR_ARG;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    void* ps[] = {b->d_runs, b->d_dyn, b->d_item0, b->d_cell0};
+    void* ps[] = {b->d_runs, b->d_dyn, b->d_item0, b->d_cell0, b->d_trust};
     for (void* q : ps)
         if (q) (void)hipFree(q);
     for (rs_ctx* c : b->runs) rs_destroy(c);

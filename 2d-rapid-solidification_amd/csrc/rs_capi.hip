@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 
 #include "rs_host.hpp"
@@ -317,6 +318,16 @@ TempArgs temp_args(rs_ctx* ctx, int64_t step) {
     return t;
 }
 
+void note_variant(rs_ctx* ctx, int v, int ident) {
+    if (v == ctx->last_variant && ident == ctx->last_ident) {
+        ++ctx->variant_run;
+    } else {
+        ctx->last_variant = v;
+        ctx->last_ident = ident;
+        ctx->variant_run = 1;
+    }
+}
+
 uint64_t splitmix64(uint64_t* s) {
     uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
@@ -469,9 +480,10 @@ int rs_context_info(rs_ctx* ctx, double* out, int32_t n) {
     if (!ctx || !out || n < 1) return RS_ERR_ARG;
     double lo = 0.0, hi = 0.0, err = 0.0;
     if (ctx->mixed) sgte_table_info(&lo, &hi, &err);
-    const double v[6] = {err, ctx->mixed ? kMixedTref : 0.0, lo, hi, (double)ctx->band_h,
-                         (double)ctx->wpb};
-    for (int q = 0; q < n && q < 6; ++q) out[q] = v[q];
+    const double v[9] = {err, ctx->mixed ? kMixedTref : 0.0, lo, hi, (double)ctx->band_h,
+                         (double)ctx->wpb, (double)ctx->last_variant, (double)ctx->last_ident,
+                         (double)ctx->variant_run};
+    for (int q = 0; q < n && q < 9; ++q) out[q] = v[q];
     return RS_OK;
 }
 
@@ -601,8 +613,30 @@ int rs_get_constants(rs_ctx* ctx, rs_constants* k) {
 
 int rs_set_constants(rs_ctx* ctx, const rs_constants* k) {
     if (!ctx || !k) return RS_ERR_ARG;
-    if (k->NxT_all != ctx->k.NxT_all || k->NyT_all != ctx->k.NyT_all)
+    // every rule is checked before anything changes: the buffers were sized
+    // for the context's coarse grid, the loop takes step % cadence, and the
+    // fine window must interpolate from interior coarse columns
+    if (k->NxT_all != ctx->k.NxT_all || k->NyT_all != ctx->k.NyT_all || k->NxT != ctx->k.NxT ||
+        k->NyT != ctx->k.NyT)
         return fail(ctx, RS_ERR_ARG, "coarse grid geometry cannot change");
+    if (k->step_fields <= 0 || k->step_tip <= 0 || k->step_check_point <= 0 || k->step_pull_back <= 0)
+        return fail(ctx, RS_ERR_ARG, "output and pull-back cadences must be positive");
+    if (k->ratio_dt_dtT < 1) return fail(ctx, RS_ERR_ARG, "ratio_dt_dtT must be at least 1");
+    if (!(k->dt > 0.0) || !std::isfinite(k->dt) || !(k->dtT > 0.0) || !std::isfinite(k->dtT))
+        return fail(ctx, RS_ERR_ARG, "dt and dtT must be positive and finite");
+    if (k->step < 0) return fail(ctx, RS_ERR_ARG, "step must be non-negative");
+    if (ctx->p.ifTFC) {
+        const int r = ctx->p.ratio_dxT_dx;
+        for (int ile : {k->i_left_end, k->i_left_end_interp})
+            if (ile - 1 < 0 || (ctx->p.Nx - 2 + ile) / r + 1 > ctx->k.NxT - 1) {
+                char b[256];
+                snprintf(b, sizeof b,
+                         "i_left_end / i_left_end_interp = %d puts the fine window off the "
+                         "temperature sample (NxT=%d)",
+                         ile, ctx->k.NxT);
+                return fail(ctx, RS_ERR_ARG, b);
+            }
+    }
     ctx->k = *k;
     ctx->d = derive_constants(ctx->p, ctx->k.dt);
     return RS_OK;
@@ -657,10 +691,13 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             const double* o = ctx->orient;
             const bool ident = o[0] == 1.0 && o[1] == 0.0 && o[3] == 0.0 && o[4] == 1.0 &&
                                o[6] == 0.0 && o[7] == 0.0;
-            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->surrogate, ctx->stream));
+            int v = 0;
+            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->surrogate, ctx->stream, &v));
+            note_variant(ctx, v, ident ? 1 : 0);
         } else {
             const StepArgsF64 a = step_args(ctx);
             HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+            note_variant(ctx, -1, 1);
         }
         if (p.at_coef != 0.0) {
             const double vdn = p.V_D > 0.0 ? p.V_D * tau0 / (p.S * p.W0) : 0.0;

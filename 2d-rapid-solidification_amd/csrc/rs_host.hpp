@@ -35,6 +35,11 @@ double tau0_value(const rs_params& p);
 struct SurrState;
 void surr_free(rs_ctx* ctx);
 bool surr_ready(const rs_ctx* ctx);
+// identity of the installed network (0 when none): equal for networks that
+// evaluate alike (weights, normalisation, window, tolerance, format)
+uint64_t surr_identity(const rs_ctx* ctx);
+// host copy of the trust bitmap (kTrustN words), nullptr when none
+const uint64_t* surr_trust_host(const rs_ctx* ctx);
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a);
 int surr_update_policy(rs_ctx* ctx);
 
@@ -73,6 +78,11 @@ struct rs_ctx {
     // network (surr_update_policy), 0 never, 1 always (RS_SURR_ALWAYS)
     int surr_always = -1;
     int64_t surr_steps = 0;  // surrogate steps taken (policy schedule)
+    // step kernel of the last rs_step launch: variant (launch_step_mixed's
+    // code; -1 = the FP64 kernel or none yet), identity-orientation flag,
+    // and how many launches in a row used that kernel
+    int last_variant = -1, last_ident = -1;
+    int64_t variant_run = 0;
     std::string err;
     // HIP-event timing of the fused step kernel [0] and the coarse T kernel
     // [1] on every timing-th step (0: off)

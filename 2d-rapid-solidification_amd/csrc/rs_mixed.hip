@@ -681,7 +681,15 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
     }
 }
 
-hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s) {
+hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
+                             int* variant) {
+    if (variant)
+        *variant = !surr                                 ? 0
+                   : a.s_fmt == kActFp16Pk               ? 4
+                   : a.s_fmt == kActFp16 && a.s_always   ? 7
+                   : a.s_fmt == kActFp16                 ? 3
+                   : a.s_fmt == kActSplit                ? 2
+                                                         : 1;
     if (wpb < 1) wpb = 1;
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);

@@ -57,7 +57,11 @@ struct StepArgsMixed {
 // built orientation, alpha0 = beta0 = gamma0 = 0), which removes the z row
 // of the rotated normal
 // surr: thermodynamics from the learned surrogate with exact fallback
-hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s);
+// *variant (optional) receives the SURR template argument of the kernel it
+// launched: 0 exact table, 1 BF16, 2 split BF16, 3 FP16 (rows without a
+// trusted cell skip the network), 4 packed FP16, 7 FP16 in every row
+hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
+                             int* variant = nullptr);
 hipError_t launch_T32_copy(const double* T, float* T32, int64_t n, double Tref, hipStream_t s);
 hipError_t launch_init_fields_f32(float* psi, double* c, const double* x0, int Nx, int Ny,
                                   double dx, int mode, double cinf, double c_liq, double c_sol,

@@ -31,6 +31,9 @@ struct SurrState {
     double T_lo = 0.0, T_hi = 0.0, tol = 1e-4;
     int fmt = kActFp16;  // inference activation format (training: BF16, split unless kActBf16)
     uint64_t h_trust[kTrustN] = {};
+    // FNV-1a hash of the FP32 weights, the normalisation, the window, the
+    // tolerance and the format: equal for networks that evaluate alike
+    uint64_t ident = 0;
     bool ready = false;
 };
 
@@ -236,6 +239,12 @@ void surr_free(rs_ctx* ctx) {
 
 bool surr_ready(const rs_ctx* ctx) { return ctx->surr && ctx->surr->ready; }
 
+uint64_t surr_identity(const rs_ctx* ctx) { return surr_ready(ctx) ? ctx->surr->ident : 0; }
+
+const uint64_t* surr_trust_host(const rs_ctx* ctx) {
+    return surr_ready(ctx) ? ctx->surr->h_trust : nullptr;
+}
+
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
     const SurrState* s = ctx->surr;
     a.mlp = fp16_fmt(s->fmt) ? &s->F->h : &s->F->f;
@@ -273,6 +282,27 @@ int surr_update_policy(rs_ctx* ctx) {
 }
 
 }  // namespace rs
+
+namespace {
+
+uint64_t fnv1a(uint64_t h, const void* data, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(data);
+    for (size_t q = 0; q < n; ++q) {
+        h ^= b[q];
+        h *= 0x100000001B3ULL;
+    }
+    return h;
+}
+
+uint64_t network_identity(const SurrState* s, const float* P) {
+    uint64_t h = fnv1a(0xCBF29CE484222325ULL, P, sizeof(float) * kNParam);
+    h = fnv1a(h, &s->norm, sizeof s->norm);
+    const double w[3] = {s->T_lo, s->T_hi, s->tol};
+    h = fnv1a(h, w, sizeof w);
+    return fnv1a(h, &s->fmt, sizeof s->fmt);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -496,6 +526,13 @@ int rs_surrogate_train(rs_ctx* ctx, const rs_surrogate_opts* o, double* report, 
     double err4[4], rmax, frac;
     rc = build_trust(ctx, err4, &rmax, &frac);
     if (rc) return rc;
+    {
+        std::vector<float> Ph(kNParam);
+        HIPCHK(ctx, hipMemcpyAsync(Ph.data(), s->P, sizeof(float) * kNParam, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        s->ident = network_identity(s, Ph.data());
+    }
     s->ready = true;
     const double rep[RS_SURR_NREPORT] = {mse1,    frac, err4[0],      err4[1], err4[2], err4[3],
                                          rmax,    ms * 1e-3, (double)t, lo,     hi,      mse0};
@@ -554,6 +591,7 @@ int rs_surrogate_set(rs_ctx* ctx, const float* params, int32_t np, const double*
     double err4[4], rmax, frac;
     rc = build_trust(ctx, err4, &rmax, &frac);
     if (rc) return rc;
+    s->ident = network_identity(s, params);
     s->ready = true;
     return RS_OK;
 }

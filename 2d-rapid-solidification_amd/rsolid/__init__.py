@@ -134,6 +134,19 @@ def ensemble_params(N: int, deltaT: float, Vp: float, precision: int = RS_PREC_M
     return p
 
 
+def draw_ensemble(nruns: int = 64, seed: int = 2024):
+    """The seeded batched-ensemble draw: grid sizes from {256, 512, 1024,
+    2048} with probabilities 0.4/0.3/0.2/0.1 (uneven counts), undercooling
+    ~ U[25, 60] K, pulling speed ~ U[0.03, 0.3] m/s and one init_config
+    seed per run. Returns (sizes, deltaT, Vp, seeds)."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([256, 512, 1024, 2048], size=nruns, p=[0.4, 0.3, 0.2, 0.1])
+    dts = rng.uniform(25.0, 60.0, nruns)
+    vps = rng.uniform(0.03, 0.3, nruns)
+    seeds = [int(x) for x in rng.integers(1, 2**31, nruns)]
+    return sizes, dts, vps, seeds
+
+
 def write_field_txt(path: str, data: np.ndarray, di: int = 1, dj: int = 1) -> None:
     a = np.ascontiguousarray(data, dtype=np.float64)
     rc = lib.rs_write_field_txt(path.encode(), _dptr(a), a.shape[1], a.shape[0], di, dj)
@@ -300,10 +313,12 @@ class Context:
         self._chk(lib.rs_set_finite_check(self._h, every), "rs_set_finite_check")
 
     def info(self) -> dict:
-        v = (C.c_double * 6)()
-        self._chk(lib.rs_context_info(self._h, v, 6), "rs_context_info")
+        v = (C.c_double * 9)()
+        self._chk(lib.rs_context_info(self._h, v, 9), "rs_context_info")
         return {"poly_err": v[0], "Tref": v[1], "T_lo": v[2], "T_hi": v[3],
-                "band_h": int(v[4]), "waves_per_block": int(v[5])}
+                "band_h": int(v[4]), "waves_per_block": int(v[5]),
+                "step_variant": int(v[6]), "step_ident": int(v[7]),
+                "step_variant_run": int(v[8])}
 
     def calibrate_copy(self, n: int, reps: int = 5) -> float:
         """Average ms of a plain FP64 copy of n doubles (destroys the fields)."""

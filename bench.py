@@ -12,9 +12,12 @@ The metric compares the surrogate + FP32 path with the plain FP64 path on
 the same 4096^2 workload, so the three paths (surrogate + FP32 fields, FP32
 fields with the exact CALPHAD table, plain FP64) are timed in the same run
 from the same initial state: `value` is the headline path's MLUPS, `paths`
-holds all three, and `tip_velocity_err` compares each path's tip advance over
-the timed steps with the plain FP64 path's. The surrogate is trained on the
-GPU before the warm-up (outside the timed region; its time is reported).
+holds all three, and `tip_velocity_err` compares each path's tip advance
+between steps 2000 and 6000 of an untimed horizon that every path runs
+before its warm-up with the plain FP64 path's. The surrogate is trained on
+the GPU before that horizon (outside the timed region; its time is
+reported). The kernel named in `roofline` is the one the library reports it
+launched, and HIP events bracket every timed launch of it.
 `ensemble` adds the batched-ensemble measurement (64 runs of uneven sizes
 and undercoolings, one persistent launch per step, HIP-graph replay,
 autotuned tiles; see run_ensemble). Prints ONE JSON line.
@@ -46,28 +49,46 @@ BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8,        # psi f64, c f64
 # algorithmic flops of one surrogate evaluation (2->64->64->4 MLP, one per
 # cell): 2 (2*64 + 64*64 + 64*4)
 FLOPS_PER_CELL_MLP = 2 * (2 * 64 + 64 * 64 + 64 * 4)
-KERNEL = {"fp64": "step_fp64_kernel", "mixed": "step_mixed_kernel<IDENT=1,SURR=0>",
-          # surrogate: the FP16-activation variant without the branch around the
-          # network, which a run switches to after its first 64 steps when
-          # fewer than 2% of the wavefront rows lack a trusted cell
-          "surrogate": "step_mixed_kernel<IDENT=1,SURR=7>"}
-TIMING_EVERY = 10
+# HIP-event pairs bracket the kernels of every timed step, so the kernel
+# average covers exactly the timed launches
+TIMING_EVERY = 1
+# untimed horizon before the warm-up, the same for every path and every
+# --steps/--warmup: the tip velocity is measured between steps TV_FROM and
+# TV_TO of it (the front has left its initial transient by TV_FROM), and the
+# surrogate path makes its per-run kernel choice inside it (after its first
+# 64 steps), so the timed steps launch the kernel the line names
+TV_FROM, TV_TO = 2000, 6000
 # surrogate activation formats (RS_ACT_*)
 ACT_FORMATS = {"bf16": 0, "bf16-split": 1, "fp16": 2, "fp16-packed": 3}
-PROFILE = {"fp64": "step_fp64_kernel_4096.md", "mixed": "step_mixed_kernel_4096.md",
-           "surrogate": "step_mixed_kernel_surrogate_4096.md"}
+PROFILES = ("step_fp64_kernel_4096.md", "step_mixed_kernel_4096.md",
+            "step_mixed_kernel_surrogate_4096.md")
 
 
-def _traffic_from_profile(path: str):
-    """HBM bytes per launch of the fused kernel as measured by rocprofv3 PMC
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
-    recorded in profiles/; None when not measured."""
-    try:
-        for line in open(os.path.join(ROOT, "profiles", PROFILE[path])):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    except OSError:
-        return None
+def kernel_name(info: dict) -> str:
+    """The step kernel the library launched last (rs_context_info)."""
+    v = info["step_variant"]
+    if v < 0:
+        return "step_fp64_kernel"
+    return f"step_mixed_kernel<IDENT={info['step_ident']},SURR={v}>"
+
+
+def _traffic_from_profile(kname: str):
+    """HBM bytes per launch of the named step kernel as measured by
+    rocprofv3 PMC (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration) and recorded in profiles/ next to the kernel's name; None
+    when no profile covers that kernel."""
+    for f in PROFILES:
+        name = traffic = None
+        try:
+            for line in open(os.path.join(ROOT, "profiles", f), encoding="utf-8"):
+                if line.startswith("kernel_name:"):
+                    name = line.split(":", 1)[1].strip()
+                elif line.startswith("traffic_bytes_per_launch:"):
+                    traffic = float(line.split(":", 1)[1])
+        except OSError:
+            continue
+        if name == kname and traffic is not None:
+            return traffic
     return None
 
 
@@ -189,14 +210,15 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     train = None
     if path == "surrogate":
         train = _surrogate_setup(ctx, train_epochs, act_format, weights)
+    # untimed horizon: tip velocity between TV_FROM and TV_TO
+    ctx.step(TV_FROM)
+    x_from = _tip_x(ctx)
+    ctx.step(TV_TO - TV_FROM)
+    x_to = _tip_x(ctx)
     ctx.step(warmup)
     ctx.sync()
-    x0 = _tip_x(ctx)
     if path == "surrogate":
         ctx.surrogate_stats(reset=True)
-    # HIP-event pairs around the kernels of every TIMING_EVERY-th timed step:
-    # the kernel averages come from the timed region while the events add
-    # little to it
     ctx.timing_enable(True, every=TIMING_EVERY)
     t0 = time.perf_counter()
     ctx.step(steps)
@@ -206,7 +228,6 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     t_ms, _ = ctx.timing(1)
     ctx.timing_enable(False)
     fb = ctx.surrogate_stats() if path == "surrogate" else None
-    x1 = _tip_x(ctx)
     kc = ctx.constants()
     info = ctx.info()
     copy_ms = ctx.calibrate_copy(grid * grid, 5) if calibrate else None
@@ -214,8 +235,9 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     cells = grid * grid
     bpc = BYTES_PER_CELL[path]
     gbs = bpc * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    kname = kernel_name(info)
     hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(path)}
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": _traffic_from_profile(kname)}
     if path == "surrogate":
         tf = FLOPS_PER_CELL_MLP * cells / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
         roof = {"bound": "mfma", "achieved": round(tf, 2), "peak": MFMA16_PEAK_TFLOPS,
@@ -224,11 +246,11 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     else:
         roof = dict(hbm)
         roof["bytes_per_cell"] = bpc
-    kname = KERNEL[path]
-    if path == "surrogate" and act_format is not None and act_format != ACT_FORMATS["fp16"]:
-        kname = f"step_mixed_kernel<IDENT=1,SURR={1 + act_format}>"
+    # every timed launch used the named kernel when the run of launches of
+    # the last kernel covers the timed steps
     roof.update({"kernel": kname, "kernel_avg_ms": round(k_ms, 5),
-                 "kernel_launches_timed": k_n, "kernel_launches": steps})
+                 "kernel_launches_timed": k_n, "kernel_launches": steps,
+                 "kernel_same_for_all_timed": info["step_variant_run"] >= steps})
     return {
         "mlups": cells * steps / el / 1e6,
         "ms_per_step": el * 1e3 / steps,
@@ -237,7 +259,7 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
         "coarse_T": [kc.NxT, kc.NyT],
         "info": info,
         "copy_ms": copy_ms,
-        "tip_advance_cells": x1 - x0,
+        "tip_advance_cells": x_to - x_from,
         "train": train,
         "fallback": fb,
     }
@@ -255,11 +277,7 @@ def run_ensemble(nruns: int, steps: int, warmup: int, seed: int, tune_steps: int
     the same run stepped alone on the plain FP64 path."""
     import numpy as np
     import rsolid
-    rng = np.random.default_rng(seed)
-    sizes = rng.choice([256, 512, 1024, 2048], size=nruns, p=[0.4, 0.3, 0.2, 0.1])
-    dts = rng.uniform(25.0, 60.0, nruns)
-    vps = rng.uniform(0.03, 0.3, nruns)
-    seeds = [int(x) for x in rng.integers(1, 2**31, nruns)]
+    sizes, dts, vps, seeds = rsolid.draw_ensemble(nruns, seed)
     prec = rsolid.RS_PREC_SURROGATE if surrogate else rsolid.RS_PREC_MIXED
     params = [rsolid.ensemble_params(int(n), float(d), float(v), prec)
               for n, d, v in zip(sizes, dts, vps)]
@@ -406,8 +424,10 @@ def main() -> int:
     out["paths"] = {q: {"value": round(r["mlups"], 2), "unit": "MLUPS",
                         "ms_per_step": round(r["ms_per_step"], 5),
                         "kernel_avg_ms": r["roofline"]["kernel_avg_ms"],
+                        "kernel": r["roofline"]["kernel"],
                         "tip_advance_cells": round(r["tip_advance_cells"], 5)}
                     for q, r in res.items()}
+    out["tip_velocity_window_steps"] = [TV_FROM, TV_TO]
     if "fp64" in res:
         ref = res["fp64"]
         for q, r in res.items():

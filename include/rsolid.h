@@ -215,7 +215,14 @@ int rs_destroy(rs_ctx* ctx);
  * out[0] = largest error (J/mol) of the mixed path's SGTE polynomial table
  *          against the FP64 routine on its window (0 on the FP64 path),
  * out[1] = Tref, out[2] = window low (K), out[3] = window high (K),
- * out[4] = band height (rows per wavefront), out[5] = waves per block. */
+ * out[4] = band height (rows per wavefront), out[5] = waves per block,
+ * out[6] = step kernel of the last rs_step launch: -1 the FP64 kernel (or
+ *          none yet), else the FP32-field kernel's thermodynamics variant
+ *          (0 exact table, 1 BF16 network, 2 split-BF16, 3 FP16 with rows
+ *          lacking a trusted cell skipping the network, 4 packed FP16,
+ *          7 FP16 evaluated in every row),
+ * out[7] = 1 if that launch used the identity-orientation kernel,
+ * out[8] = how many launches in a row, up to the last, used that kernel. */
 int rs_context_info(rs_ctx* ctx, double* out, int32_t n);
 /* never NULL; with ctx NULL: why the calling thread's last rs_create failed
  * (which rule its parameters broke, or the device error) */

@@ -60,6 +60,8 @@ def _load() -> C.CDLL:
         "or_common_tangent": (C.c_int, [P(RsParams), d, P(d), P(d)]),
         "or_at_flux": (d, [P(RsParams), d, d, d, d, d, d, d, d, d, d, d, d]),
         "or_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
+        "or_interpolate_at": (C.c_int, [vp, C.c_int]),
+        "or_integrate_phi_at": (C.c_int, [vp, P(d), P(d), C.c_int, P(d)]),
         "or_set_threads": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -217,6 +219,23 @@ class State:
 
     def solute_mass(self) -> float:
         return lib.or_solute_mass(self._h)
+
+    def interpolate_at(self, i_left_end: int) -> np.ndarray:
+        """T_fine (Ny x Nx) interpolated from the current coarse T with the
+        given i_left_end (interpolate_temperature)."""
+        self._chk(lib.or_interpolate_at(self._h, i_left_end), "interpolate_at")
+        return self.get(abi.RS_FIELD_TFINE)
+
+    def integrate_phi_at(self, psi: np.ndarray, psi_next: np.ndarray, i_left_end: int) -> np.ndarray:
+        """Latent-heat block sums (NyT_all x NxT_all) of psi_next - psi
+        (integrate_phi); replaces the state's psi and psi_next."""
+        a = np.ascontiguousarray(psi, dtype=np.float64)
+        b = np.ascontiguousarray(psi_next, dtype=np.float64)
+        k = self.constants()
+        out = np.zeros((k.NyT_all, k.NxT_all))
+        self._chk(lib.or_integrate_phi_at(self._h, _dptr(a), _dptr(b), i_left_end, _dptr(out)),
+                  "integrate_phi_at")
+        return out
 
     def tip(self) -> dict:
         t = RsTipRow()

@@ -697,6 +697,25 @@ static void or_integrate_phi(or_state* s, int i_left_end) {
     }
 }
 
+/* test entry points of the two routines above: T_fine from the current
+ * coarse T with the given i_left_end, and the latent-heat block sums of
+ * (psi_next - psi) for given arrays (Nx*Ny each) into out (NxT_all*NyT_all) */
+int or_interpolate_at(or_state* s, int i_left_end) {
+    if (!s) return RS_ERR_ARG;
+    or_interpolate(s, i_left_end);
+    return RS_OK;
+}
+
+int or_integrate_phi_at(or_state* s, const double* psi, const double* psi_next, int i_left_end,
+                        double* out) {
+    if (!s || !psi || !psi_next || !out) return RS_ERR_ARG;
+    memcpy(s->psi, psi, sizeof(double) * (size_t)s->n);
+    memcpy(s->psi_next, psi_next, sizeof(double) * (size_t)s->n);
+    or_integrate_phi(s, i_left_end);
+    memcpy(out, s->dphidt, sizeof(double) * (size_t)s->nT);
+    return RS_OK;
+}
+
 /* compute_temperature (fun.cu:1537-1578), race-free boundary semantics */
 static void or_compute_temperature(or_state* s, int64_t step) {
     const rs_params* p = &s->p;

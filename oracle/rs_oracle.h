@@ -89,6 +89,15 @@ int or_common_tangent(const rs_params* p, double T, double* cl, double* cs);
 double or_at_flux(const rs_params* p, double vdn, double dt, double pP, double pQ, double nP,
                   double nQ, double cP, double cQ, double t1, double t2, double t3, double t4);
 
+/* interpolate_temperature (fun.cu:1580-1608) of the current coarse T into
+ * T_fine with the given i_left_end (read back as RS_FIELD_TFINE). */
+int or_interpolate_at(or_state* s, int i_left_end);
+/* integrate_phi (fun.cu:1494-1535): latent-heat block sums 0.5 * sum of
+ * (psi_next - psi) for the given Nx*Ny arrays, written to out
+ * (NxT_all*NyT_all). Replaces the state's psi and psi_next. */
+int or_integrate_phi_at(or_state* s, const double* psi, const double* psi_next, int i_left_end,
+                        double* out);
+
 /* splitmix64 stream used by the seeded configuration initialiser. */
 uint64_t or_splitmix64(uint64_t* state);
 

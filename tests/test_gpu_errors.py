@@ -81,6 +81,64 @@ def test_pull_back_off_the_temperature_sample_is_rejected(rs, off):
     ctx.close()
 
 
+def _bad_constants(k0, rule):
+    k = type(k0).from_buffer_copy(k0)
+    if rule == "NyT":
+        k.NyT += 1
+    elif rule == "NxT":
+        k.NxT -= 1
+    elif rule == "step_pull_back":
+        k.step_pull_back = 0
+    elif rule == "step_tip":
+        k.step_tip = -1
+    elif rule == "step_fields":
+        k.step_fields = 0
+    elif rule == "step_check_point":
+        k.step_check_point = 0
+    elif rule == "ratio_dt_dtT":
+        k.ratio_dt_dtT = 0
+    elif rule == "dt":
+        k.dt = float("nan")
+    elif rule == "dtT":
+        k.dtT = -1.0
+    elif rule == "step":
+        k.step = -3
+    elif rule == "i_left_end":
+        k.i_left_end = 0
+    elif rule == "i_left_end_interp":
+        k.i_left_end_interp = k.NxT * 10
+    return k
+
+
+RULES = ["NyT", "NxT", "step_pull_back", "step_tip", "step_fields", "step_check_point",
+         "ratio_dt_dtT", "dt", "dtT", "step", "i_left_end", "i_left_end_interp"]
+
+
+def test_set_constants_rejects_each_broken_rule(rs):
+    """rs_set_constants checks every rule before it changes anything: the
+    coarse grid the buffers were sized for, positive cadences (the loop
+    takes step % cadence), ratio_dt_dtT >= 1, positive finite dt and dtT, a
+    non-negative step and a fine window inside the temperature sample. Each
+    rejection is host-side (no kernel runs) and leaves the state as it was."""
+    p = rs.config_params(256, True)
+    say("set_constants rules")
+    ctx = rs.Context(p)
+    ctx.init_config(3)
+    ctx.step(5)
+    before = _state(ctx)
+    k0 = ctx.constants()
+    for rule in RULES:
+        with pytest.raises(RsError) as e:
+            ctx.set_constants(_bad_constants(k0, rule))
+        say(rule, "->", str(e.value))
+        assert e.value.code == RS_ERR_ARG, rule
+        _same(before, _state(ctx))
+    ctx.set_constants(k0)
+    ctx.step(3)
+    assert ctx.constants().step == 8
+    ctx.close()
+
+
 def test_surrogate_step_without_weights_is_a_state_error(rs):
     p = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
     say("surrogate without weights")
