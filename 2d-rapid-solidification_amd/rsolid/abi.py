@@ -115,3 +115,31 @@ EXPORTED = [
     "rs_write_checkpoint", "rs_read_checkpoint", "rs_write_parameters", "rs_read_parameters",
     "rs_download_columns", "rs_save_state", "rs_load_state",
 ]
+
+
+# ---- include/rs_peritectic.h (3D peritectic Ti-Ag stepper) ----------------
+TP_FIELD_PHI1, TP_FIELD_PHI2, TP_FIELD_C, TP_FIELD_MU = 0, 1, 2, 3
+
+
+class TpParams(C.Structure):
+    _fields_ = [
+        ("Nx", _I), ("Ny", _I), ("Nz", _I), ("zcut_from_top", _I),
+        ("DT", _D), ("DXin", _D), ("DYin", _D), ("DZin", _D),
+        ("a1", _D), ("a2", _D), ("a3", _D), ("b", _D), ("lambda_", _D), ("alpha", _D),
+        ("Da", _D), ("Db", _D), ("A1", _D), ("A2", _D), ("B11", _D), ("B21", _D), ("B12", _D),
+        ("B22", _D), ("T", _D), ("R", _D), ("theta", _D), ("ncaps", _I), ("reserved", _I),
+        ("cap_x", _D * 4), ("cap_y", _D * 4),
+    ]
+
+    def copy(self) -> "TpParams":
+        q = TpParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(TpParams))
+        return q
+
+
+TP_EXPORTED = [
+    "tp_abi_info", "tp_params_reference", "tp_derived", "tp_create", "tp_destroy", "tp_last_error",
+    "tp_init_reference", "tp_upload", "tp_download", "tp_step", "tp_get_step", "tp_set_step",
+    "tp_report", "tp_sync", "tp_timing", "tp_write_cross_section", "tp_write_break_point",
+    "tp_read_break_point",
+]

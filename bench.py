@@ -167,6 +167,59 @@ def _surrogate_setup(ctx, train_epochs, act_format, weights):
     return rep
 
 
+PERITECTIC_SLAB = (512, 512, 128)
+# algorithmic HBM bytes per cell of one fused peritectic step: read and
+# write phi1, phi2, c (FP64); mu is recomputed, never stored
+PERITECTIC_BYTES_PER_CELL = 6 * 8
+
+
+def run_peritectic(steps: int, warmup: int, cpu_steps: int = 0) -> dict:
+    """The 3D three-phase peritectic Ti-Ag model on one GPU's z-slab of the
+    reference's 4-GPU run (512 x 512 x 128 planes, its two-nuclei start),
+    FP64: step throughput, the fused step kernel's HIP-event time and its
+    HBM roofline, and the CPU restatement on the host cores beside it."""
+    from rsolid import peritectic as tp
+    Nx, Ny, Nz = PERITECTIC_SLAB
+    p = tp.reference_params(Nx, Ny, Nz)
+    g = tp.Slab(p)
+    g.init_reference()
+    g.step(warmup)
+    g.sync()
+    g.timing(reset=True)
+    t0 = time.perf_counter()
+    g.step(steps)
+    g.sync()
+    el = time.perf_counter() - t0
+    k_ms, k_n = g.timing()
+    rep = g.report(20)
+    g.close()
+    cells = Nx * Ny * Nz
+    gbs = PERITECTIC_BYTES_PER_CELL * cells / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    out = {"value": round(cells * steps / el / 1e6, 2), "unit": "MLUPS", "grid": [Nx, Ny, Nz],
+           "steps": steps, "warmup": warmup, "ms_per_step": round(el * 1e3 / steps, 5),
+           "dtype": "f64", "mean_c": rep["mean_c"],
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "traffic": _traffic_from_profile("tp::step_kernel<false>"),
+                        "kernel": "tp::step_kernel<false>", "kernel_avg_ms": round(k_ms, 5),
+                        "kernel_launches_timed": k_n,
+                        "bytes_per_cell": PERITECTIC_BYTES_PER_CELL}}
+    if cpu_steps > 0:
+        import oracle
+        o = oracle.TpState(p)
+        o.init_reference()
+        o.step(1)
+        t0 = time.perf_counter()
+        o.step(cpu_steps)
+        el = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": round(cells * cpu_steps / el / 1e6, 3), "unit": "MLUPS",
+                               "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                               "sample": f"CPU FP64 restatement, {Nx}x{Ny}x{Nz}, {cpu_steps} steps "
+                                         f"({el:.1f} s)"}
+    return out
+
+
 def run_antitrapping(grid: int, steps: int, warmup: int) -> dict:
     """The antitrapping / solute-trapping extension (a_t = 1/(2 sqrt 2),
     V_D = 2 Vp) on the FP64 and FP32 paths of the same workload: step
@@ -379,6 +432,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--antitrapping", action="store_true",
                     help="with --only: also time the antitrapping extension section")
+    ap.add_argument("--peritectic", action="store_true",
+                    help="time only the 3D peritectic Ti-Ag slab (512 x 512 x 128) and print "
+                         "its JSON object")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
                          "(byte-counter calibration for rocprofv3 --pmc)")
@@ -387,6 +443,11 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the y-band decomposition is a design note, not code).", file=sys.stderr)
         return 2
+
+    if a.peritectic:
+        print(json.dumps(run_peritectic(a.steps, a.warmup,
+                                        0 if a.no_cpu_baseline else 3)), flush=True)
+        return 0
 
     order = [a.path] + ([] if a.only else [q for q in ("surrogate", "mixed", "fp64") if q != a.path])
     res = {}
@@ -453,6 +514,8 @@ def main() -> int:
             out["surrogate"]["weights_loaded_from"] = r["train"]["loaded_from"]
     if not a.only or a.antitrapping:
         out["antitrapping"] = run_antitrapping(a.grid, 300, 20)
+    if not a.only:
+        out["peritectic_3d"] = run_peritectic(200, 10, 0 if a.no_cpu_baseline else 3)
     if a.ensemble_runs > 0 and not a.only:
         out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
                                        act_format=ACT_FORMATS[a.act_format])

@@ -22,6 +22,7 @@ abi = importlib.util.module_from_spec(_spec)
 sys.modules["rs_oracle_abi"] = abi
 _spec.loader.exec_module(abi)
 RS_FIELD_T, RsConstants, RsParams, RsTipRow = abi.RS_FIELD_T, abi.RsConstants, abi.RsParams, abi.RsTipRow
+TpParams = abi.TpParams
 
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
@@ -61,6 +62,17 @@ def _load() -> C.CDLL:
         "or_at_flux": (d, [P(RsParams), d, d, d, d, d, d, d, d, d, d, d, d]),
         "or_splitmix64": (C.c_uint64, [P(C.c_uint64)]),
         "or_interpolate_at": (C.c_int, [vp, C.c_int]),
+        "tpo_g": (d, [d, d, d]), "tpo_gp1": (d, [d, d, d]), "tpo_gp2": (d, [d, d, d]),
+        "tpo_mu": (d, [P(TpParams), d, d, d]),
+        "tpo_bulk_force": (None, [P(TpParams), d, d, d, P(d)]),
+        "tpo_create": (vp, [P(TpParams)]),
+        "tpo_destroy": (None, [vp]),
+        "tpo_init_reference": (C.c_int, [vp]),
+        "tpo_get": (C.c_int, [vp, C.c_int, P(d), C.c_int64]),
+        "tpo_put": (C.c_int, [vp, C.c_int, P(d), C.c_int64]),
+        "tpo_step": (C.c_int, [vp, C.c_int64]),
+        "tpo_get_step": (C.c_int64, [vp]),
+        "tpo_report": (C.c_int, [vp, C.c_int, P(d)]),
         "or_integrate_phi_at": (C.c_int, [vp, P(d), P(d), C.c_int, P(d)]),
         "or_set_threads": (C.c_int, [C.c_int]),
     }
@@ -241,3 +253,60 @@ class State:
         t = RsTipRow()
         lib.or_tip(self._h, C.byref(t))
         return t.as_dict()
+
+
+# ---- 3D peritectic Ti-Ag restatement (tp_oracle.c) -------------------------
+def tp_g(pi, pj, pk): return lib.tpo_g(pi, pj, pk)
+def tp_gp1(pi, pj, pk): return lib.tpo_gp1(pi, pj, pk)
+def tp_gp2(pi, pj, pk): return lib.tpo_gp2(pi, pj, pk)
+def tp_mu(p, p1, p2, c): return lib.tpo_mu(C.byref(_cast(p, TpParams)), p1, p2, c)
+
+
+def tp_bulk_force(p, p1, p2, muu) -> np.ndarray:
+    out = np.zeros(3)
+    lib.tpo_bulk_force(C.byref(_cast(p, TpParams)), p1, p2, muu, _dptr(out))
+    return out
+
+
+class TpState:
+    """CPU restatement of one Ti-Ag slab (fields of shape (Nz, Ny, Nx))."""
+
+    def __init__(self, p):
+        self.p = _cast(p, TpParams)
+        self._h = lib.tpo_create(C.byref(self.p))
+        if not self._h:
+            raise RuntimeError("tpo_create failed")
+        self.shape = (self.p.Nz, self.p.Ny, self.p.Nx)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.tpo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def init_reference(self):
+        lib.tpo_init_reference(self._h)
+
+    def get(self, field: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        if lib.tpo_get(self._h, field, _dptr(out), out.size):
+            raise RuntimeError("tpo_get failed")
+        return out
+
+    def put(self, field: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if lib.tpo_put(self._h, field, _dptr(a), a.size):
+            raise RuntimeError("tpo_put failed")
+
+    def step(self, n: int = 1):
+        lib.tpo_step(self._h, n)
+
+    def get_step(self) -> int:
+        return lib.tpo_get_step(self._h)
+
+    def report(self, k_check: int = 20) -> dict:
+        out = np.zeros(2)
+        lib.tpo_report(self._h, k_check, _dptr(out))
+        return {"mean_c": out[0], "max_abs_c_plane": out[1]}

@@ -7,7 +7,8 @@ import re
 
 import pytest
 
-from rsolid.abi import EXPORTED, RS_ABI_VERSION, RsConstants, RsParams, RsTipRow
+from rsolid.abi import (EXPORTED, RS_ABI_VERSION, TP_EXPORTED, RsConstants, RsParams, RsTipRow,
+                        TpParams)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -23,6 +24,22 @@ def test_every_declared_symbol_is_exported(rs):
     assert sorted(EXPORTED) == decl
     for name in decl:
         assert hasattr(rs.lib, name), name
+
+
+def test_peritectic_header_symbols_are_exported(rs):
+    """include/rs_peritectic.h (the 3D Ti-Ag stepper): every declared
+    function is in the library and in the Python mirror; tp_params has the
+    mirror's layout."""
+    import rsolid.peritectic  # noqa: F401  (binds the signatures)
+    txt = open(os.path.join(ROOT, "include", "rs_peritectic.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    decl = sorted(set(re.findall(r"\b(tp_[a-z_0-9]+)\s*\(", txt)))
+    assert sorted(TP_EXPORTED) == decl
+    for name in decl:
+        assert hasattr(rs.lib, name), name
+    n = C.c_int32()
+    assert rs.lib.tp_abi_info(C.byref(n)) == 0
+    assert n.value == C.sizeof(TpParams)
 
 
 def test_struct_layouts(rs):
