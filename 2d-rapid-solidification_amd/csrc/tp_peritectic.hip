@@ -7,7 +7,7 @@
 // cell move through HBM (mu written and read back). Here it is one launch:
 // mu is a pointwise function of (phi1, phi2, c), so the kernel evaluates it
 // for the cells it loads and never stores it (48 B per cell: read and write
-// phi1, phi2, c). A workgroup owns a 64 x 4 column tile and marches up a run
+// phi1, phi2, c). A workgroup owns a 64 x 8 column tile and marches up a run
 // of z planes: the current plane's phi1, phi2 and mu (tile plus a one-cell
 // ring) sit in LDS for the x/y neighbours, the planes below and above in
 // registers for the z neighbours. Ghost cells are written by the thread
@@ -130,7 +130,7 @@ __device__ __forceinline__ void update(const Coef& k, double p1, double p2, doub
     oc = c + (k.DT * k.alpha) * ((((hxp - hxn) * k.DXin) + ((hyp - hyn) * k.DYin)) + ((hzp - hzn) * k.DZin));
 }
 
-constexpr int kBX = 64, kBY = 4;              // output tile (one wavefront per row)
+constexpr int kBX = 64, kBY = 8;              // output tile (one wavefront per row)
 constexpr int kLX = kBX + 2, kLY = kBY + 2;   // with the one-cell ring
 constexpr int kRing = 2 * kLX + 2 * kBY;      // ring cells, one per thread tid < kRing
 
@@ -181,7 +181,7 @@ __device__ __forceinline__ void store_cell(const StepArgs& a, int i, int j, int 
     const int xs1 = i == a.Nx - 2 ? 0 : (i == 1 ? a.Nx - 1 : -1);
     const int ys1 = j == a.Ny - 2 ? 0 : (j == 1 ? a.Ny - 1 : -1);
     const int zs1 = k == a.Nz - 2 ? a.Nz - 1 : (k == 1 ? 0 : -1);
-    const size_t o = ((size_t)k * a.Ny + j) * a.Nx + i;
+    const int o = (k * a.Ny + j) * a.Nx + i;
     a.n1[o] = o1;
     a.n2[o] = o2;
     a.nc[o] = oc;
@@ -190,15 +190,50 @@ __device__ __forceinline__ void store_cell(const StepArgs& a, int i, int j, int 
     for (int q = 1; q < 8; ++q) {
         const int x = xs[q & 1], y = ys[(q >> 1) & 1], z = zs[q >> 2];
         if (x < 0 || y < 0 || z < 0) continue;
-        const size_t g = ((size_t)z * a.Ny + y) * a.Nx + x;
+        const int g = (z * a.Ny + y) * a.Nx + x;
         a.n1[g] = o1;
         a.n2[g] = o2;
         a.nc[g] = oc;
     }
 }
 
+// phi1, phi2, c of a cell as stored (zeros outside the grid: such cells
+// are never read by an interior cell's stencil). 32-bit cell offsets: the
+// grid holds fewer than 2^31 cells (checked at tp_create).
+__device__ __forceinline__ void raw_load(const StepArgs& a, int i, int j, int k, double& q1,
+                                         double& q2, double& qc) {
+    if (i < 0 || i >= a.Nx || j < 0 || j >= a.Ny) {
+        q1 = q2 = qc = 0.0;
+        return;
+    }
+    const int o = (k * a.Ny + j) * a.Nx + i;
+    q1 = a.p1[o];
+    q2 = a.p2[o];
+    qc = a.c[o];
+}
+
+// mu of a cell whose stored values are (q1, q2, qc): a ghost takes its
+// image's mu, read here (ghosts are rare: the grid's faces)
+__device__ __forceinline__ double mu_at(const StepArgs& a, int i, int j, int k, double q1, double q2,
+                                        double qc) {
+    if (i < 0 || i >= a.Nx || j < 0 || j >= a.Ny) return 0.0;
+    const int ii = img_p(i, a.Nx), jj = img_p(j, a.Ny), kk = img_z(k, a.Nz);
+    if (ii != i || jj != j || kk != k) {
+        const int m = (kk * a.Ny + jj) * a.Nx + ii;
+        q1 = a.p1[m];
+        q2 = a.p2[m];
+        qc = a.c[m];
+    }
+    return mu_of(a.k, q1, q2, qc);
+}
+
+// Software pipeline over the planes: while plane k is updated from the LDS
+// copy of plane k and the own column's planes k - 1 and k + 1 in registers,
+// the loads of plane k + 2 are in flight; plane k + 1's mu and LDS copy are
+// finished at the top of the next trip, one plane after its loads were
+// issued. One barrier per plane (double-buffered LDS).
 template <bool MOB23>
-__global__ __launch_bounds__(kBX * kBY) void step_kernel(StepArgs a) {
+__global__ __launch_bounds__(kBX * kBY, 2) void step_kernel(StepArgs a) {
     __shared__ double s1[2][kLY][kLX], s2[2][kLY][kLX], sm[2][kLY][kLX];
     const int L = rs::xcd_order(blockIdx.x, gridDim.x);
     const int txt = L % a.ntx, rest = L / a.ntx;
@@ -226,34 +261,44 @@ __global__ __launch_bounds__(kBX * kBY) void step_kernel(StepArgs a) {
     const int hi = txt * kBX + hx - 1, hj = tyt * kBY + hy - 1;
     const bool inter = i >= 1 && i <= a.Nx - 2 && j >= 1 && j <= a.Ny - 2;
 
-    double m1, m2, mm, dummy;  // own cell, plane k - 1
-    double c1, c2, cc, cm;     // own cell, plane k
-    load_cell(a, i, j, k0 - 1, m1, m2, dummy, mm);
-    load_cell(a, i, j, k0, c1, c2, cc, cm);
+    double m1, m2, mc, mm;  // own cell, plane k - 1
+    double c1, c2, cc, cm;  // own cell, plane k
+    double p1, p2, pc;      // own cell, plane k + 1 (as loaded)
+    double h1 = 0.0, h2 = 0.0, hc = 0.0;  // ring cell, plane k + 1 (as loaded)
+    raw_load(a, i, j, k0 - 1, m1, m2, mc);
+    mm = mu_at(a, i, j, k0 - 1, m1, m2, mc);
+    raw_load(a, i, j, k0, c1, c2, cc);
+    cm = mu_at(a, i, j, k0, c1, c2, cc);
     s1[0][ty + 1][tx + 1] = c1;
     s2[0][ty + 1][tx + 1] = c2;
     sm[0][ty + 1][tx + 1] = cm;
     if (ring) {
-        double q1, q2, qc, qm;
-        load_cell(a, hi, hj, k0, q1, q2, qc, qm);
+        double q1, q2, qc;
+        raw_load(a, hi, hj, k0, q1, q2, qc);
         s1[0][hy][hx] = q1;
         s2[0][hy][hx] = q2;
-        sm[0][hy][hx] = qm;
+        sm[0][hy][hx] = mu_at(a, hi, hj, k0, q1, q2, qc);
+        raw_load(a, hi, hj, k0 + 1, h1, h2, hc);
     }
+    raw_load(a, i, j, k0 + 1, p1, p2, pc);
     __syncthreads();
     int b = 0;
     for (int k = k0; k < k1; ++k) {
-        double p1, p2, pc, pm;  // own cell, plane k + 1
-        load_cell(a, i, j, k + 1, p1, p2, pc, pm);
+        // plane k + 1: mu of the own and ring cells, its LDS copy
+        const double pm = mu_at(a, i, j, k + 1, p1, p2, pc);
         s1[b ^ 1][ty + 1][tx + 1] = p1;
         s2[b ^ 1][ty + 1][tx + 1] = p2;
         sm[b ^ 1][ty + 1][tx + 1] = pm;
         if (ring) {
-            double q1, q2, qc, qm;
-            load_cell(a, hi, hj, k + 1, q1, q2, qc, qm);
-            s1[b ^ 1][hy][hx] = q1;
-            s2[b ^ 1][hy][hx] = q2;
-            sm[b ^ 1][hy][hx] = qm;
+            s1[b ^ 1][hy][hx] = h1;
+            s2[b ^ 1][hy][hx] = h2;
+            sm[b ^ 1][hy][hx] = mu_at(a, hi, hj, k + 1, h1, h2, hc);
+        }
+        // plane k + 2 in flight during the update of plane k
+        double n1 = 0.0, n2 = 0.0, nc = 0.0;
+        if (k + 2 <= k1) {
+            raw_load(a, i, j, k + 2, n1, n2, nc);
+            if (ring) raw_load(a, hi, hj, k + 2, h1, h2, hc);
         }
         if (inter) {
             Nb nb[6];
@@ -276,6 +321,9 @@ __global__ __launch_bounds__(kBX * kBY) void step_kernel(StepArgs a) {
         c2 = p2;
         cc = pc;
         cm = pm;
+        p1 = n1;
+        p2 = n2;
+        pc = nc;
     }
 }
 
@@ -471,8 +519,8 @@ int check(const tp_params& p, std::string* why) {
         *why = "grid must be at least 4 x 4 x 4 (one interior cell between the ghosts on each axis)";
         return RS_ERR_ARG;
     }
-    if ((int64_t)p.Nx * p.Ny * p.Nz > ((int64_t)1 << 33)) {
-        *why = "grid too large";
+    if ((int64_t)p.Nx * p.Ny * p.Nz >= ((int64_t)1 << 31)) {
+        *why = "grid must hold fewer than 2^31 cells (32-bit cell offsets)";
         return RS_ERR_ARG;
     }
     if (p.ncaps < 0 || p.ncaps > 4) {

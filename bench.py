@@ -61,7 +61,7 @@ TV_FROM, TV_TO = 2000, 6000
 # surrogate activation formats (RS_ACT_*)
 ACT_FORMATS = {"bf16": 0, "bf16-split": 1, "fp16": 2, "fp16-packed": 3}
 PROFILES = ("step_fp64_kernel_4096.md", "step_mixed_kernel_4096.md",
-            "step_mixed_kernel_surrogate_4096.md")
+            "step_mixed_kernel_surrogate_4096.md", "tp_step_kernel_512x512x128.md")
 
 
 def kernel_name(info: dict) -> str:
@@ -173,7 +173,7 @@ PERITECTIC_SLAB = (512, 512, 128)
 PERITECTIC_BYTES_PER_CELL = 6 * 8
 
 
-def run_peritectic(steps: int, warmup: int, cpu_steps: int = 0) -> dict:
+def run_peritectic(steps: int, warmup: int, cpu_steps: int = 0, calibrate: bool = False) -> dict:
     """The 3D three-phase peritectic Ti-Ag model on one GPU's z-slab of the
     reference's 4-GPU run (512 x 512 x 128 planes, its two-nuclei start),
     FP64: step throughput, the fused step kernel's HIP-event time and its
@@ -204,6 +204,14 @@ def run_peritectic(steps: int, warmup: int, cpu_steps: int = 0) -> dict:
                         "kernel": "tp::step_kernel<false>", "kernel_avg_ms": round(k_ms, 5),
                         "kernel_launches_timed": k_n,
                         "bytes_per_cell": PERITECTIC_BYTES_PER_CELL}}
+    if calibrate:
+        import rsolid
+        ctx = rsolid.Context(rsolid.config_params(4096, True))
+        nb = 2 * 8 * 4096 * 4096
+        ms = ctx.calibrate_copy(4096 * 4096, 5)
+        ctx.close()
+        out["copy_calibration"] = {"bytes_per_launch": nb, "avg_ms": round(ms, 5),
+                                   "GBps": round(nb / (ms * 1e-3) / 1e9, 1)}
     if cpu_steps > 0:
         import oracle
         o = oracle.TpState(p)
@@ -445,8 +453,8 @@ def main() -> int:
         return 2
 
     if a.peritectic:
-        print(json.dumps(run_peritectic(a.steps, a.warmup,
-                                        0 if a.no_cpu_baseline else 3)), flush=True)
+        print(json.dumps(run_peritectic(a.steps, a.warmup, 0 if a.no_cpu_baseline else 30,
+                                        a.calibrate_copy)), flush=True)
         return 0
 
     order = [a.path] + ([] if a.only else [q for q in ("surrogate", "mixed", "fp64") if q != a.path])
@@ -515,7 +523,7 @@ def main() -> int:
     if not a.only or a.antitrapping:
         out["antitrapping"] = run_antitrapping(a.grid, 300, 20)
     if not a.only:
-        out["peritectic_3d"] = run_peritectic(200, 10, 0 if a.no_cpu_baseline else 3)
+        out["peritectic_3d"] = run_peritectic(200, 10, 0 if a.no_cpu_baseline else 30)
     if a.ensemble_runs > 0 and not a.only:
         out["ensemble"] = run_ensemble(a.ensemble_runs, a.ensemble_steps, 20, 2024, 5,
                                        act_format=ACT_FORMATS[a.act_format])

@@ -5,7 +5,7 @@ The update has no transcendental function and both sides round every
 operation in the reference's order without contraction, so the bar is
 bitwise equality of phi1, phi2, c and mu after every run, on the
 reference's one-GPU slab (512 x 512 x 128, its two-nuclei start), on a
-small box, on ragged shapes that are not multiples of the 64 x 4 tile or
+small box, on ragged shapes that are not multiples of the 64 x 8 tile or
 of the z segment, with the phase-2/3 mobilities and the a1 term switched
 on, and from uploaded fields whose ghosts are not their images."""
 import os
