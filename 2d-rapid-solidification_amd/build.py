@@ -42,9 +42,10 @@ PER_FILE = {
     # the Ti-Ag stepper rounds every operation as written: it reproduces
     # the CPU restatement bit for bit
     "tp_peritectic.hip": ["-ffp-contract=off"],
+    "rs_noalloc.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
-           "rs_capi.hip", "rs_surr.hip", "rs_batch.hip", "rs_io.hip", "tp_peritectic.hip"]
+           "rs_capi.hip", "rs_surr.hip", "rs_batch.hip", "rs_io.hip", "tp_peritectic.hip", "rs_noalloc.hip"]
 
 LIB = os.path.join(PKG, "rsolid", "librsolid.so")
 DRIVER_SRC = os.path.join(PKG, "driver", "rsolid_run.cpp")

@@ -373,6 +373,7 @@ int finite_check(rs_ctx* ctx) {
 }  // namespace
 
 namespace rs {
+void set_thread_error(const std::string& why) { g_create_err = why; }
 StepArgsMixed mixed_step_args(rs_ctx* ctx) { return step_args_mixed(ctx); }
 TempArgs coarse_T_args(rs_ctx* ctx, int64_t step) { return temp_args(ctx, step); }
 double tau0_value(const rs_params& p) { return tau0_of(p); }

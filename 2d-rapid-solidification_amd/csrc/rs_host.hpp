@@ -26,6 +26,9 @@ int precise_tip(const rs_params& p, const double* psi, TipFit* out);
 // writes them): no field download
 int precise_tip_rows(const rs_params& p, const int* irow, const double* vals, TipFit* out);
 
+// the message rs_last_error(NULL) returns on this thread (rs_capi.hip)
+void set_thread_error(const std::string& why);
+
 // step arguments of a context at its current state (rs_capi.hip)
 StepArgsMixed mixed_step_args(rs_ctx* ctx);
 TempArgs coarse_T_args(rs_ctx* ctx, int64_t step);

@@ -55,6 +55,7 @@ def _load() -> C.CDLL:
         "rs_tip_column": (C.c_int, [vp, P(C.c_int32)]),
         "rs_pull_back": (C.c_int, [vp, C.c_int32]),
         "rs_solute_mass": (C.c_int, [vp, P(C.c_double)]),
+        "rs_solute_mass_noalloc": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp]),
         "rs_tip": (C.c_int, [vp, P(RsTipRow)]),
         "rs_sync": (C.c_int, [vp]),
         "rs_timing_enable": (C.c_int, [vp, C.c_int32]),

@@ -224,8 +224,9 @@ int rs_destroy(rs_ctx* ctx);
  * out[7] = 1 if that launch used the identity-orientation kernel,
  * out[8] = how many launches in a row, up to the last, used that kernel. */
 int rs_context_info(rs_ctx* ctx, double* out, int32_t n);
-/* never NULL; with ctx NULL: why the calling thread's last rs_create failed
- * (which rule its parameters broke, or the device error) */
+/* never NULL; with ctx NULL: why the calling thread's last rs_create or
+ * rs_solute_mass_noalloc failed (which rule its arguments broke, or the
+ * device error) */
 const char* rs_last_error(const rs_ctx* ctx);
 int rs_device_name(char* buf, int32_t len);
 
@@ -254,6 +255,19 @@ int rs_step(rs_ctx* ctx, int64_t nsteps);
 int rs_tip_column(rs_ctx* ctx, int32_t* i_tip);            /* get_itip_all_grain */
 int rs_pull_back(rs_ctx* ctx, int32_t i_offset);            /* set_pull_back+pull_back */
 int rs_solute_mass(rs_ctx* ctx, double* mass);              /* sum of c, interior */
+/* The same reduction without a context and without any allocation. No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops). c: device
+ * pointer to an Nx x Ny FP64 field (row-major, ghosts included); mass:
+ * device pointer receiving the interior sum, bitwise the value
+ * rs_solute_mass gives for the same field; stream: a hipStream_t (NULL =
+ * the null stream). Enqueues two kernels and returns without waiting. The
+ * per-row partial sums live in a static device array of 65536 doubles
+ * (512 KiB per GPU, held until the process ends): Ny > 65536 returns
+ * RS_ERR_ARG. Every call on one GPU shares that array, so calls that
+ * overlap on a GPU race; the first call on a device looks up the array's
+ * address there, so a capturing caller makes one call before capturing. */
+int rs_solute_mass_noalloc(const double* c, int32_t Nx, int32_t Ny, double* mass, void* stream);
 int rs_tip(rs_ctx* ctx, rs_tip_row* tip);                       /* set_tip_output */
 int rs_sync(rs_ctx* ctx);
 
