@@ -46,7 +46,10 @@ struct rs_batch {
 
 namespace {
 
+// waits for the batch stream before returning an error (no pending copy may
+// outlive a host buffer of the failed call)
 int bfail(rs_batch* b, int code, const char* what, hipError_t e = hipSuccess) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     char buf[512];
     if (e != hipSuccess)
         snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));

@@ -23,8 +23,12 @@ int create_fail(int code, const std::string& why) {
     return code;
 }
 
+// Every error return waits for the context stream first: a call that
+// enqueued a copy from one of its own host buffers and then failed must not
+// return (and free that buffer) while the copy is still pending.
 int fail(rs_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
     if (ctx) {
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         char buf[512];
         if (e != hipSuccess)
             snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));

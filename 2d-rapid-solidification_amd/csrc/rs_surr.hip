@@ -43,7 +43,10 @@ namespace {
 
 constexpr double kTiBreak = 1941.0;  // G_Ti piece boundary, K
 
+// waits for the context stream before returning an error (see rs_capi.hip:
+// no pending copy may outlive a host buffer of the failed call)
 int fail(rs_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     char buf[512];
     if (e != hipSuccess)
         snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));

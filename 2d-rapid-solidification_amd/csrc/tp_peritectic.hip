@@ -454,6 +454,7 @@ thread_local std::string g_tp_create_err;
 
 int tfail(tp_ctx* ctx, int code, const char* what, hipError_t e = hipSuccess) {
     if (ctx) {
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
         char buf[512];
         if (e != hipSuccess)
             snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
