@@ -77,7 +77,7 @@ int build_tables(rs_batch* b) {
     b->h_runs.resize(n);
     b->h_dyn.resize(n);
     std::vector<int> item0(n + 1, 0), cell0(n + 1, 0);
-    bool ident = true, at_any = false;
+    bool ident = true, at_any = false, led_any = false;
     int nsub = 0;
     // the persistent kernel stages one network in LDS for every run: the
     // runs must share it, and each keeps its own trust bound
@@ -109,6 +109,7 @@ int build_tables(rs_batch* b) {
             surr_step_args(c, R.a);
             if (surr) R.a.trust = b->d_trust;
         }
+        led_any = led_any || R.a.led != nullptr;
         R.a.band_h = b->band_h[r];
         R.a.nbands = (p.Ny - 2 + R.a.band_h - 1) / R.a.band_h;
         for (int q = 0; q < 2; ++q) {
@@ -175,6 +176,9 @@ int build_tables(rs_batch* b) {
     L.ident = ident;
     L.at_any = at_any ? 1 : 0;
     L.nsub = nsub;
+    // a run that started its ledger since the capture needs the recording kernel
+    if (L.led_any != (led_any ? 1 : 0)) b->dirty = true;
+    L.led_any = led_any ? 1 : 0;
     const rs_ctx* c0 = b->runs[0];
     L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
     if (L.surr) L.surr = 1 + b->h_runs[0].a.s_fmt;

@@ -104,6 +104,8 @@ void free_all(rs_ctx* ctx) {
     if (ctx->tiprow) (void)hipFree(ctx->tiprow);
     if (ctx->outi) (void)hipFree(ctx->outi);
     if (ctx->finite_out) (void)hipFree(ctx->finite_out);
+    if (ctx->led) (void)hipFree(ctx->led);
+    if (ctx->led_acc) (void)hipFree(ctx->led_acc);
     if (ctx->h_pin) (void)hipHostFree(ctx->h_pin);
     for (int q = 0; q < 3; ++q)
         for (hipEvent_t e : ctx->ev[q]) (void)hipEventDestroy(e);
@@ -160,7 +162,20 @@ StepArgsF64 step_args(rs_ctx* ctx) {
     a.an.lap_w_diag = p.lap_w_diag;
     a.an.inv2dx = 1.0 / (2.0 * p.dx);
     a.an.dx = p.dx;
+    a.led = ctx->led_on ? ctx->led : nullptr;
     return a;
+}
+
+// Solute ledger reset: clears the edge records and the pull-back sum, and
+// takes the current interior mass as the reference (nothing while the
+// ledger is off).
+int ledger_reset(rs_ctx* ctx) {
+    if (!ctx->led_on) return RS_OK;
+    HIPCHK(ctx, hipMemsetAsync(ctx->led, 0, sizeof(double) * 2 * (size_t)(ctx->p.Ny + ctx->p.Nx), ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->led_acc, 0, sizeof(double), ctx->stream));
+    HIPCHK(ctx, launch_mass(ctx->c[ctx->cur], ctx->p.Nx, ctx->p.Ny, ctx->rowd, ctx->led_acc + 1,
+                            ctx->stream));
+    return RS_OK;
 }
 
 // A shift that would move the fine window off the coarse T sample is
@@ -177,6 +192,10 @@ int do_pull_back(rs_ctx* ctx, int off) {
     k.i_offset = off;
     k.i_offset_history += off;
     const int c0 = ctx->cur;
+    // solute ledger: the interior mass before and after the shift
+    if (ctx->led_on)
+        HIPCHK(ctx, launch_mass(ctx->c[c0], ctx->p.Nx, ctx->p.Ny, ctx->rowd, ctx->led_acc + 2,
+                                ctx->stream));
     if (ctx->mixed)
         HIPCHK(ctx, launch_pull_back_f32(ctx->psi32[c0], ctx->c[c0], ctx->psi32[c0 ^ 1],
                                          ctx->c[c0 ^ 1], ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf,
@@ -184,6 +203,11 @@ int do_pull_back(rs_ctx* ctx, int off) {
     else
         HIPCHK(ctx, launch_pull_back(ctx->psi[c0], ctx->c[c0], ctx->psi[c0 ^ 1], ctx->c[c0 ^ 1],
                                      ctx->p.Nx, ctx->p.Ny, off, ctx->p.cinf, ctx->stream));
+    if (ctx->led_on) {
+        HIPCHK(ctx, launch_mass(ctx->c[c0 ^ 1], ctx->p.Nx, ctx->p.Ny, ctx->rowd, ctx->led_acc + 3,
+                                ctx->stream));
+        HIPCHK(ctx, launch_ledger_pull(ctx->led_acc + 2, ctx->led_acc, ctx->stream));
+    }
     ctx->cur ^= 1;
     return RS_OK;
 }
@@ -260,6 +284,7 @@ StepArgsMixed step_args_mixed(rs_ctx* ctx) {
     a.dL0 = (float)(p.L0_liq - p.L0_bcc);
     a.dL1 = (float)(p.L1_liq - p.L1_bcc);
     a.rk = rk_of(p);
+    a.led = ctx->led_on ? ctx->led : nullptr;
     return a;
 }
 
@@ -285,6 +310,8 @@ int init_common(rs_ctx* ctx, const std::vector<double>& x0, int mode) {
         int rc = refresh_T32(ctx);
         if (rc) return rc;
     }
+    int rc = ledger_reset(ctx);
+    if (rc) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->k.i_left_end_interp = ctx->k.i_left_end;
     return check_window(ctx, ctx->k.i_left_end);
@@ -378,6 +405,7 @@ int finite_check(rs_ctx* ctx) {
 
 namespace rs {
 void set_thread_error(const std::string& why) { g_create_err = why; }
+int solute_ledger_reset(rs_ctx* ctx) { return ledger_reset(ctx); }
 StepArgsMixed mixed_step_args(rs_ctx* ctx) { return step_args_mixed(ctx); }
 TempArgs coarse_T_args(rs_ctx* ctx, int64_t step) { return temp_args(ctx, step); }
 double tau0_value(const rs_params& p) { return tau0_of(p); }
@@ -458,6 +486,8 @@ int rs_create(const rs_params* p, rs_ctx** out) {
               hipMalloc(&ctx->outd, sizeof(double) * 8) == hipSuccess &&
               hipMalloc(&ctx->outi, sizeof(int) * 8) == hipSuccess &&
               hipMalloc(&ctx->x0, sizeof(double) * p->Ny) == hipSuccess &&
+              hipMalloc(&ctx->led, sizeof(double) * 2 * (size_t)(p->Ny + p->Nx)) == hipSuccess &&
+              hipMalloc(&ctx->led_acc, sizeof(double) * 4) == hipSuccess &&
               hipHostMalloc(&ctx->h_pin, 64) == hipSuccess;
     if (!ok) {
         free_all(ctx);
@@ -470,6 +500,8 @@ int rs_create(const rs_params* p, rs_ctx** out) {
          hipMemsetAsync(ctx->T[0], 0, tb, ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->T[1], 0, tb, ctx->stream) == hipSuccess &&
          hipMemsetAsync(ctx->lh, 0, lhb, ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->led, 0, sizeof(double) * 2 * (size_t)(p->Ny + p->Nx), ctx->stream) == hipSuccess &&
+         hipMemsetAsync(ctx->led_acc, 0, sizeof(double) * 4, ctx->stream) == hipSuccess &&
          hipStreamSynchronize(ctx->stream) == hipSuccess;
     if (!ok) {
         free_all(ctx);
@@ -570,6 +602,10 @@ int rs_upload(rs_ctx* ctx, int32_t field, const double* host, int64_t count) {
     }
     default:
         return fail(ctx, RS_ERR_ARG, "field cannot be uploaded");
+    }
+    if (field == RS_FIELD_C) {  // a new composition field: the ledger starts over
+        const int rc = ledger_reset(ctx);
+        if (rc) return rc;
     }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return RS_OK;
@@ -756,6 +792,37 @@ int rs_solute_mass(rs_ctx* ctx, double* mass) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_pin, ctx->outd, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     *mass = ctx->h_pin[0];
+    return RS_OK;
+}
+
+int rs_solute_ledger(rs_ctx* ctx, double* out, int32_t n, int32_t reset) {
+    if (!ctx || !out || n < 1) return RS_ERR_ARG;
+    if (!ctx->led_on) {
+        if (!reset) return fail(ctx, RS_ERR_STATE, "the solute ledger is off (start it with reset != 0)");
+        // start: the record begins at the current fields
+        ctx->led_on = true;
+        const int rc = ledger_reset(ctx);
+        if (rc) return rc;
+    }
+    const int Ny = ctx->p.Ny, Nx = ctx->p.Nx;
+    // outd: [0] mass now, [1..2] x faces, [3..4] y faces
+    HIPCHK(ctx, launch_mass(ctx->c[ctx->cur], Nx, Ny, ctx->rowd, ctx->outd, ctx->stream));
+    HIPCHK(ctx, launch_sum(ctx->led, 1, Ny - 2, ctx->outd + 1, ctx->stream));
+    HIPCHK(ctx, launch_sum(ctx->led + Ny, 1, Ny - 2, ctx->outd + 2, ctx->stream));
+    HIPCHK(ctx, launch_sum(ctx->led + 2 * Ny, 1, Nx - 2, ctx->outd + 3, ctx->stream));
+    HIPCHK(ctx, launch_sum(ctx->led + 2 * Ny + Nx, 1, Nx - 2, ctx->outd + 4, ctx->stream));
+    double h[5], acc[2];
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->outd, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(acc, ctx->led_acc, sizeof acc, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const double ex = h[1] + h[2], ey = h[3] + h[4];
+    const double v[6] = {h[0], acc[1], acc[0], ex, ey, ((h[0] - acc[1]) - acc[0]) - (ex + ey)};
+    for (int q = 0; q < n && q < 6; ++q) out[q] = v[q];
+    if (reset) {
+        const int rc = ledger_reset(ctx);
+        if (rc) return rc;
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     return RS_OK;
 }
 

@@ -26,6 +26,8 @@ int precise_tip(const rs_params& p, const double* psi, TipFit* out);
 // writes them): no field download
 int precise_tip_rows(const rs_params& p, const int* irow, const double* vals, TipFit* out);
 
+// restarts the context's solute ledger at the current fields (rs_capi.hip)
+int solute_ledger_reset(rs_ctx* ctx);
 // the message rs_last_error(NULL) returns on this thread (rs_capi.hip)
 void set_thread_error(const std::string& why);
 
@@ -73,6 +75,13 @@ struct rs_ctx {
     double* x0 = nullptr;
     double* tfine = nullptr;
     double* h_pin = nullptr;  // pinned scalars for D2H
+    // solute ledger: boundary-face gains per row (x faces, 2 Ny) and per
+    // column (y faces, 2 Nx), and led_acc[0] the summed mass change of the
+    // pull-backs, [1] the mass at the last reset, [2..3] a pull-back's mass
+    // before / after
+    double* led = nullptr;
+    double* led_acc = nullptr;
+    bool led_on = false;  // recording: from the first rs_solute_ledger call with reset
     double* tipvals = nullptr;  // per-row tip-fit data (8 Ny) and columns (Ny)
     int* tiprow = nullptr;
     int cur = 0, tcur = 0;

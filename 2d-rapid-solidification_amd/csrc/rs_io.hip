@@ -380,7 +380,10 @@ int rs_load_state(rs_ctx* ctx, const char* path) {
     ctx->k = kk;
     ctx->cur = 0;
     ctx->tcur = 0;
-    return RS_OK;
+    // the loaded composition field: the solute ledger starts over
+    rc = solute_ledger_reset(ctx);
+    if (rc) return rc;
+    return hipStreamSynchronize(ctx->stream) == hipSuccess ? RS_OK : RS_ERR_HIP;
 }
 
 }  // extern "C"

@@ -78,6 +78,8 @@ __global__ __launch_bounds__(256) void step_fp64_kernel(StepArgsF64 a) {
     const int jb = 1 + band * a.band_h;
     const int je = min(jb + a.band_h - 1, a.Ny - 2);
     const int Nx = a.Nx;
+    // the solute ledger's x faces: does this wavefront own column 1 or Nx-2
+    const bool led_wave = a.led && __ballot(own && (i == 1 || i == Nx - 2)) != 0;
 
     // rows jb-1 and jb
     Cell m = make_cell(a, a.psi[(int64_t)(jb - 1) * Nx + ic], a.c[(int64_t)(jb - 1) * Nx + ic], ic,
@@ -134,6 +136,14 @@ __global__ __launch_bounds__(256) void step_fp64_kernel(StepArgsF64 a) {
             const double F0m = 0.25 * (S00 + S0m + Shn + Snn) * (m.mu - f00) * a.mu_den;
             const double dc = a.kdc * (F10 + F01 + Fm0 + F0m) / a.dx2;
             const double cn = fmax(z.c + dc, 1.0e-6);
+            // ledger: the flux through the boundary faces (ghost side); the
+            // test is wavefront-uniform, taken by the edge tiles and rows only
+            if (led_wave || (a.led && (j == 1 || j == a.Ny - 2))) {
+                if (i == 1) a.led[j] += a.kdc * Fm0 / a.dx2;
+                if (i == Nx - 2) a.led[a.Ny + j] += a.kdc * F10 / a.dx2;
+                if (j == 1) a.led[2 * a.Ny + i] += a.kdc * F0m / a.dx2;
+                if (j == a.Ny - 2) a.led[2 * a.Ny + Nx + i] += a.kdc * F01 / a.dx2;
+            }
 
             // ---- stores + ghost images ----
             const int64_t row = (int64_t)j * Nx;
@@ -352,6 +362,20 @@ hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* 
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(256), 0, s, rowbuf, 1, Ny - 2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum(const double* v, int lo, int hi, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_rows_kernel, dim3(1), dim3(256), 0, s, v, lo, hi, out);
+    return hipGetLastError();
+}
+
+__global__ void ledger_pull_kernel(const double* m, double* acc) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) acc[0] += m[1] - m[0];
+}
+
+hipError_t launch_ledger_pull(const double* m, double* acc, hipStream_t s) {
+    hipLaunchKernelGGL(ledger_pull_kernel, dim3(1), dim3(64), 0, s, m, acc);
     return hipGetLastError();
 }
 

@@ -53,6 +53,11 @@ struct StepArgsF64 {
     double dt, AA, q_solid, dG_den, mu_den, kdc, dx2;
     RK rk;
     Aniso<double> an;
+    // solute ledger (or null): the solute the edge cells gain through the
+    // boundary faces, accumulated over the steps (led[j] x = 1/2 face of row
+    // j, led[Ny + j] x = Nx - 3/2 face, led[2 Ny + i] y = 1/2 face of column
+    // i, led[2 Ny + Nx + i] y = Ny - 3/2 face)
+    double* led;
 };
 
 struct TempArgs {
@@ -147,6 +152,10 @@ hipError_t launch_mass(const double* c, int Nx, int Ny, double* rowbuf, double* 
                        hipStream_t s);
 hipError_t launch_interp(const TempSrc& ts, double* Tfine, int Nx, int Ny, hipStream_t s);
 hipError_t launch_copy(const double* a, double* b, int64_t n, hipStream_t s);
+// fixed-order FP64 sum of v[lo..hi] into out[0] (one workgroup)
+hipError_t launch_sum(const double* v, int lo, int hi, double* out, hipStream_t s);
+// solute ledger: acc[0] += m[1] - m[0] (the mass change of one pull-back)
+hipError_t launch_ledger_pull(const double* m, double* acc, hipStream_t s);
 // number of non-finite values of f[0, n) and the smallest index holding one
 // (out[0] += count, out[1] = min(out[1], index); out must be initialised to
 // 0 and ~0)

@@ -479,7 +479,10 @@ __device__ __forceinline__ SurrLds surr_lds(StepLds<SURR>& L) {
 // scalar registers already hold the network or run state, load through
 // plain pointers too: measured 3% faster there. Both address the same
 // elements.
-template <bool IDENT, int SURR, bool BUF = true>
+// LED: the solute ledger runs (a.led set): the boundary faces' fluxes are
+// recorded. A template flag, not a branch: the record costs registers in
+// these register-limited kernels even when it is skipped (4% of a step).
+template <bool IDENT, int SURR, bool BUF = true, bool LED = false>
 __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* lntab,
                                           const SurrLds& sl, int tile_x, int band, unsigned& nfb,
                                           unsigned& nall, const TPatch& tp) {
@@ -571,6 +574,12 @@ __device__ __forceinline__ void step_tile(const StepArgsMixed& a, const double* 
             }
             const double dc = a.K * ((Fx - Fxm) + (Fy - Fy_prev));
             const double cn = fmax(Z.q.c + dc, 1.0e-6);
+            if constexpr (LED) {  // the flux through the boundary faces (ghost side)
+                if (i == 1) a.led[j] -= a.K * Fxm;
+                if (i == Nx - 2) a.led[a.Ny + j] += a.K * Fx;
+                if (j == 1) a.led[2 * a.Ny + i] -= a.K * Fy_prev;
+                if (j == a.Ny - 2) a.led[2 * a.Ny + Nx + i] += a.K * Fy;
+            }
 
             float* prow = a.psi_n + (size_t)j * Nx;
             double* crow = a.c_n + (size_t)j * Nx;
@@ -657,7 +666,7 @@ __device__ __forceinline__ TPatch tpatch_load(const StepArgsMixed& a, float* buf
 // VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
 // 4096^2 step takes 0.191 ms instead of 0.166 ms. The surrogate variants
 // hold the MFMA operands and run at 2 (3 spills 52 registers).
-template <bool IDENT, int SURR>
+template <bool IDENT, int SURR, bool LED>
 __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsMixed a) {
     __shared__ StepLds<SURR> L;
     // the coarse T' patch pays in the surrogate kernels (2%); the exact-table
@@ -673,12 +682,20 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
         tp = tpatch_load(a, tpatch, g % ntx, (g / ntx) * blockDim.y, blockDim.y, tid, nthr);
     __syncthreads();
     unsigned nfb = 0, nall = 0;
-    step_tile<IDENT, SURR, SURR == 0>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
-                                      (g / ntx) * blockDim.y + threadIdx.y, nfb, nall, tp);
+    step_tile<IDENT, SURR, SURR == 0, LED>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
+                                           (g / ntx) * blockDim.y + threadIdx.y, nfb, nall, tp);
     if constexpr (SURR != 0) {
         if (threadIdx.x == 0 && nall > 0)
             add_counts(a.fb_count + (size_t)(g % kCountSlots) * kCountStride, nfb, nall);
     }
+}
+
+template <bool IDENT, int SURR>
+static void launch_mk(dim3 grid, dim3 block, hipStream_t s, const StepArgsMixed& a) {
+    if (a.led)
+        hipLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, true>), grid, block, 0, s, a);
+    else
+        hipLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, false>), grid, block, 0, s, a);
 }
 
 hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
@@ -696,34 +713,34 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
     if (surr && a.s_fmt == kActFp16Pk) {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 4>), grid, block, 0, s, a);
+            launch_mk<true, 4>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 4>), grid, block, 0, s, a);
+            launch_mk<false, 4>(grid, block, s, a);
     } else if (surr && a.s_fmt == kActFp16 && a.s_always) {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 7>), grid, block, 0, s, a);
+            launch_mk<true, 7>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 7>), grid, block, 0, s, a);
+            launch_mk<false, 7>(grid, block, s, a);
     } else if (surr && a.s_fmt == kActFp16) {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 3>), grid, block, 0, s, a);
+            launch_mk<true, 3>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 3>), grid, block, 0, s, a);
+            launch_mk<false, 3>(grid, block, s, a);
     } else if (surr && a.s_fmt == kActSplit) {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 2>), grid, block, 0, s, a);
+            launch_mk<true, 2>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 2>), grid, block, 0, s, a);
+            launch_mk<false, 2>(grid, block, s, a);
     } else if (surr) {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 1>), grid, block, 0, s, a);
+            launch_mk<true, 1>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 1>), grid, block, 0, s, a);
+            launch_mk<false, 1>(grid, block, s, a);
     } else {
         if (ident)
-            hipLaunchKernelGGL((step_mixed_kernel<true, 0>), grid, block, 0, s, a);
+            launch_mk<true, 0>(grid, block, s, a);
         else
-            hipLaunchKernelGGL((step_mixed_kernel<false, 0>), grid, block, 0, s, a);
+            launch_mk<false, 0>(grid, block, s, a);
     }
     return hipGetLastError();
 }
@@ -752,7 +769,7 @@ __device__ __forceinline__ int batch_run_of(const int* pre, int nruns, int q) {
 // wavefronts per SIMD: the per-item run state spills at 4 (64-run ensemble:
 // 3 waves 0.425 ms per step, 4 waves with 10 spilled registers 0.49-0.52 ms);
 // plain pointers measured faster than buffer views here (0.49-0.51 ms).
-template <bool IDENT, int SURR>
+template <bool IDENT, int SURR, bool LED>
 __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaunch L) {
     __shared__ StepLds<SURR> lds;
     load_step_lds<SURR>(lds, L.runs[0].a.mlp, L.runs[0].a.trust,
@@ -778,8 +795,13 @@ __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaun
         const int tx = local % R.ntx;
         const int bg = local / R.ntx;
         unsigned nfb = 0, nall = 0;
-        step_tile<IDENT, SURR, false>(a, lds.lntab, surr_lds<SURR>(lds), tx, bg * blockDim.y + threadIdx.y,
-                               nfb, nall, TPatch{});
+        // runs without a ledger take the plain tile (a.led is null for them)
+        if (LED && a.led)
+            step_tile<IDENT, SURR, false, true>(a, lds.lntab, surr_lds<SURR>(lds), tx,
+                                                bg * blockDim.y + threadIdx.y, nfb, nall, TPatch{});
+        else
+            step_tile<IDENT, SURR, false, false>(a, lds.lntab, surr_lds<SURR>(lds), tx,
+                                                 bg * blockDim.y + threadIdx.y, nfb, nall, TPatch{});
         if constexpr (SURR != 0) {
             if (threadIdx.x == 0 && nall > 0)
                 add_counts(a.fb_count + (size_t)(item % kCountSlots) * kCountStride, nfb, nall);
@@ -844,8 +866,12 @@ __global__ __launch_bounds__(256) void batch_antitrap_kernel(BatchLaunch L) {
 
 template <bool IDENT, int SURR>
 static hipError_t batch_step_t(const BatchLaunch& L, hipStream_t s) {
-    hipLaunchKernelGGL((batch_step_kernel<IDENT, SURR>), dim3(L.step_blocks), dim3(kWave, L.wpb),
-                       0, s, L);
+    if (L.led_any)
+        hipLaunchKernelGGL((batch_step_kernel<IDENT, SURR, true>), dim3(L.step_blocks),
+                           dim3(kWave, L.wpb), 0, s, L);
+    else
+        hipLaunchKernelGGL((batch_step_kernel<IDENT, SURR, false>), dim3(L.step_blocks),
+                           dim3(kWave, L.wpb), 0, s, L);
     return hipGetLastError();
 }
 
@@ -879,26 +905,29 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
 }
 
 template <bool IDENT, int SURR>
-static int resident_t(int wpb) {
+static int resident_t(int wpb, bool led) {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step_kernel<IDENT, SURR>,
-                                                     kWave * wpb, 0) != hipSuccess)
-        return 1;
+    const hipError_t e =
+        led ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step_kernel<IDENT, SURR, true>,
+                                                           kWave * wpb, 0)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_step_kernel<IDENT, SURR, false>,
+                                                           kWave * wpb, 0);
+    if (e != hipSuccess) return 1;
     return n > 0 ? n : 1;
 }
 
 int batch_step_resident(const BatchLaunch& L) {
     switch (L.surr * 2 + (L.ident ? 1 : 0)) {
-    case 0: return resident_t<false, 0>(L.wpb);
-    case 1: return resident_t<true, 0>(L.wpb);
-    case 2: return resident_t<false, 1>(L.wpb);
-    case 3: return resident_t<true, 1>(L.wpb);
-    case 4: return resident_t<false, 2>(L.wpb);
-    case 5: return resident_t<true, 2>(L.wpb);
-    case 6: return resident_t<false, 3>(L.wpb);
-    case 7: return resident_t<true, 3>(L.wpb);
-    case 8: return resident_t<false, 4>(L.wpb);
-    default: return resident_t<true, 4>(L.wpb);
+    case 0: return resident_t<false, 0>(L.wpb, L.led_any != 0);
+    case 1: return resident_t<true, 0>(L.wpb, L.led_any != 0);
+    case 2: return resident_t<false, 1>(L.wpb, L.led_any != 0);
+    case 3: return resident_t<true, 1>(L.wpb, L.led_any != 0);
+    case 4: return resident_t<false, 2>(L.wpb, L.led_any != 0);
+    case 5: return resident_t<true, 2>(L.wpb, L.led_any != 0);
+    case 6: return resident_t<false, 3>(L.wpb, L.led_any != 0);
+    case 7: return resident_t<true, 3>(L.wpb, L.led_any != 0);
+    case 8: return resident_t<false, 4>(L.wpb, L.led_any != 0);
+    default: return resident_t<true, 4>(L.wpb, L.led_any != 0);
     }
 }
 

@@ -51,6 +51,9 @@ struct StepArgsMixed {
     float y_mean[4], y_std[4];
     int s_fmt;                 // activation format (kActBf16 ... kActFp16Pk)
     int s_always;              // FP16 format: evaluate the network in every row (no skip test)
+    // solute ledger (or null), as in StepArgsF64: boundary-face gains per
+    // row (x faces) and per column (y faces)
+    double* led;
 };
 
 // ident: the crystal orientation is the identity (the reference's only
@@ -110,6 +113,7 @@ struct BatchLaunch {
     int surr;           // 0 exact, else 1 + the surrogate's activation format
     int at_any;         // some run has the antitrapping correction on
     int nsub;           // coarse-T sub-steps per step: the largest ratio_dt_dtT of a coupled run
+    int led_any;        // some run keeps a solute ledger (its BatchRun's args carry led)
 };
 // one step of every run: fused step kernel, antitrapping pass (runs with
 // it on), coarse T kernel once per sub-step, state advance

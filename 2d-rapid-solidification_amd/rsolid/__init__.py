@@ -19,6 +19,7 @@ from .abi import (  # noqa: F401  (re-exported)
     RsSurrogateOpts, RsTipRow,
 )
 
+LEDGER_KEYS = ("mass", "mass0", "pull_back", "edge_x", "edge_y", "unaccounted")
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "librsolid.so")
 
 
@@ -56,6 +57,7 @@ def _load() -> C.CDLL:
         "rs_pull_back": (C.c_int, [vp, C.c_int32]),
         "rs_solute_mass": (C.c_int, [vp, P(C.c_double)]),
         "rs_solute_mass_noalloc": (C.c_int, [vp, C.c_int32, C.c_int32, vp, vp]),
+        "rs_solute_ledger": (C.c_int, [vp, P(C.c_double), C.c_int32, C.c_int32]),
         "rs_tip": (C.c_int, [vp, P(RsTipRow)]),
         "rs_sync": (C.c_int, [vp]),
         "rs_timing_enable": (C.c_int, [vp, C.c_int32]),
@@ -294,6 +296,16 @@ class Context:
         v = C.c_double()
         self._chk(lib.rs_solute_mass(self._h, C.byref(v)), "rs_solute_mass")
         return v.value
+
+    def solute_ledger(self, reset: bool = False) -> dict:
+        """Interior solute mass now and at the ledger's reset, the mass the
+        pull-backs changed, the solute gained through the x and through the
+        y boundary faces, and the unaccounted remainder (rs_solute_ledger).
+        The ledger is off until a call with reset=True starts it at the
+        current fields; reset=False before that raises (RS_ERR_STATE)."""
+        v = (C.c_double * 6)()
+        self._chk(lib.rs_solute_ledger(self._h, v, 6, 1 if reset else 0), "rs_solute_ledger")
+        return dict(zip(LEDGER_KEYS, list(v)))
 
     def tip(self) -> dict:
         t = RsTipRow()

@@ -114,6 +114,7 @@ EXPORTED = [
     "rs_write_field_txt", "rs_write_tip_row", "rs_append_offset", "rs_append_history",
     "rs_write_checkpoint", "rs_read_checkpoint", "rs_write_parameters", "rs_read_parameters",
     "rs_download_columns", "rs_save_state", "rs_load_state", "rs_solute_mass_noalloc",
+    "rs_solute_ledger",
 ]
 
 

@@ -255,6 +255,25 @@ int rs_step(rs_ctx* ctx, int64_t nsteps);
 int rs_tip_column(rs_ctx* ctx, int32_t* i_tip);            /* get_itip_all_grain */
 int rs_pull_back(rs_ctx* ctx, int32_t i_offset);            /* set_pull_back+pull_back */
 int rs_solute_mass(rs_ctx* ctx, double* mass);              /* sum of c, interior */
+/* Solute ledger (no counterpart in the reference, which has no mass
+ * metric). Off at rs_create: the first call must pass reset != 0, which
+ * starts it at the current fields (a call with reset == 0 before that
+ * returns RS_ERR_STATE); once on it stays on, and the step kernels record
+ * the boundary-face fluxes (the FP32-field kernels then run a variant that
+ * records them, about 4% slower). It accounts for every change of the
+ * interior solute mass since the last call with reset != 0, the context's
+ * initialisation or the last upload of c. out (up to 6 values): [0] interior mass now, [1] interior mass
+ * at the reset, [2] summed mass change of the pull-backs (the shifted-out
+ * columns and the injected cinf, fun.cu:1479-1486), [3] solute gained through
+ * the two x boundary faces (x = 1/2 and Nx - 3/2), [4] through the two y
+ * boundary faces (the ghost cells copy c and psi from their images but see
+ * their own T, fun.cu:1220-1230, 1585-1606, so these faces carry a flux;
+ * with frozen T only the x faces do), [5] unaccounted change
+ * [0] - [1] - [2] - [3] - [4]: the interior faces cancel, so this is the
+ * rounding of the c updates plus what the c >= 1e-6 clamp (fun.cu:1464)
+ * added, if it bound anywhere. Runs advanced by rs_batch_step are
+ * accounted too. */
+int rs_solute_ledger(rs_ctx* ctx, double* out, int32_t n, int32_t reset);
 /* The same reduction without a context and without any allocation. No
  * counterpart in the reference; it is for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops). c: device

@@ -75,6 +75,7 @@ def _load() -> C.CDLL:
         "tpo_report": (C.c_int, [vp, C.c_int, P(d)]),
         "or_integrate_phi_at": (C.c_int, [vp, P(d), P(d), C.c_int, P(d)]),
         "or_set_threads": (C.c_int, [C.c_int]),
+        "or_solute_ledger": (C.c_int, [vp, P(d), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -248,6 +249,12 @@ class State:
         self._chk(lib.or_integrate_phi_at(self._h, _dptr(a), _dptr(b), i_left_end, _dptr(out)),
                   "integrate_phi_at")
         return out
+
+    def solute_ledger(self, reset: bool = False) -> dict:
+        out = np.zeros(7)
+        lib.or_solute_ledger(self._h, _dptr(out), 1 if reset else 0)
+        return dict(zip(("mass", "mass0", "pull_back", "edge_x", "edge_y", "clamped",
+                         "unaccounted"), out.tolist()))
 
     def tip(self) -> dict:
         t = RsTipRow()

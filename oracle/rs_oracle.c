@@ -27,6 +27,11 @@ struct or_state {
     double *c, *phi, *psi, *c_next, *phi_next, *psi_next, *T_fine;
     double *T, *T_next, *dphidt;
     double orient[9];
+    /* solute ledger: per-row x-edge face gains (left Ny, right Ny) and
+     * clamp counts, the summed pull-back mass change, the reference mass */
+    double* led;
+    int* led_clamp;
+    double led_pull, led_m0;
 };
 
 int or_set_threads(int n) {
@@ -305,6 +310,9 @@ or_state* or_create(const rs_params* p) {
         if (!*g[q]) { or_destroy(s); return NULL; }
     }
     for (int q = 0; q < 9; q++) s->orient[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    s->led = (double*)calloc((size_t)2 * (p->Ny + p->Nx), sizeof(double));
+    s->led_clamp = (int*)calloc((size_t)p->Ny, sizeof(int));
+    if (!s->led || !s->led_clamp) { or_destroy(s); return NULL; }
     return s;
 }
 
@@ -312,6 +320,7 @@ void or_destroy(or_state* s) {
     if (!s) return;
     free(s->c); free(s->phi); free(s->psi); free(s->c_next); free(s->phi_next);
     free(s->psi_next); free(s->T_fine); free(s->T); free(s->T_next); free(s->dphidt);
+    free(s->led); free(s->led_clamp);
     free(s);
 }
 
@@ -376,6 +385,13 @@ static int or_check_window(or_state* s, int i_left_end) {
     return RS_OK;
 }
 
+static void or_ledger_reset(or_state* s) {
+    memset(s->led, 0, sizeof(double) * 2 * (size_t)(s->p.Ny + s->p.Nx));
+    memset(s->led_clamp, 0, sizeof(int) * (size_t)s->p.Ny);
+    s->led_pull = 0.0;
+    s->led_m0 = or_solute_mass(s);
+}
+
 /* initial (fun.cu:983-1021) */
 int or_init_reference(or_state* s) {
     const int Nx = s->p.Nx, Ny = s->p.Ny;
@@ -392,6 +408,7 @@ int or_init_reference(or_state* s) {
         }
     }
     or_orientation(s->orient, s->p.alpha0, s->p.beta0, s->p.gamma0);
+    or_ledger_reset(s);
     if (s->p.ifTFC) {
         or_init_T(s);
         or_interpolate(s, s->k.i_left_end);
@@ -433,6 +450,7 @@ int or_init_config(or_state* s, uint64_t seed) {
     }
     free(x0);
     or_orientation(s->orient, s->p.alpha0, s->p.beta0, s->p.gamma0);
+    or_ledger_reset(s);
     if (s->p.ifTFC) {
         or_init_T(s);
         or_interpolate(s, s->k.i_left_end);
@@ -464,6 +482,7 @@ int or_put(or_state* s, int field, const double* in, int64_t n) {
     if (!f || n != m || !in) return RS_ERR_ARG;
     memcpy(f, in, sizeof(double) * (size_t)n);
     if (field == RS_FIELD_T) memcpy(s->T_next, in, sizeof(double) * (size_t)n);
+    if (field == RS_FIELD_C) or_ledger_reset(s);
     return RS_OK;
 }
 int or_get_constants(or_state* s, rs_constants* k) { *k = s->k; return RS_OK; }
@@ -599,6 +618,14 @@ static void or_compute_c(or_state* s) {
             double F0m = 0.25 * (S00 + S0m + Shn + Snn) * (f0m - f00) * denom;
             double dc = dt * p->Dl / (p->Gamma * p->muk0) * (F10 + F01 + Fm0 + F0m) / (p->dx * p->dx);
             s->c_next[POS(s, i, j)] = fmax(cc[4] + dc, 1.0e-6);
+            /* ledger: the flux through the x boundary faces, the clamp */
+            if (i == 1) s->led[j] += dt * p->Dl / (p->Gamma * p->muk0) * Fm0 / (p->dx * p->dx);
+            if (i == Nx - 2) s->led[Ny + j] += dt * p->Dl / (p->Gamma * p->muk0) * F10 / (p->dx * p->dx);
+            /* the y faces: one row per thread, so the column entries of rows
+             * 1 and Ny - 2 are written by one thread each */
+            if (j == 1) s->led[2 * Ny + i] += dt * p->Dl / (p->Gamma * p->muk0) * F0m / (p->dx * p->dx);
+            if (j == Ny - 2) s->led[2 * Ny + Nx + i] += dt * p->Dl / (p->Gamma * p->muk0) * F01 / (p->dx * p->dx);
+            if (cc[4] + dc < 1.0e-6) s->led_clamp[j]++;
         }
     }
     or_ghosts(s, s->c_next);
@@ -765,6 +792,7 @@ int or_tip_column(or_state* s) {
 /* set_pull_back state update (fun.cu:697-704) + pull_back (fun.cu:1469-1492) */
 int or_pull_back(or_state* s, int i_offset) {
     const int Nx = s->p.Nx, Ny = s->p.Ny;
+    const double m_before = or_solute_mass(s);
     s->k.i_offset = i_offset;
     s->k.i_offset_history += i_offset;
     if (s->p.ifTFC)
@@ -785,6 +813,7 @@ int or_pull_back(or_state* s, int i_offset) {
     t = s->c; s->c = s->c_next; s->c_next = t;
     t = s->phi; s->phi = s->phi_next; s->phi_next = t;
     t = s->psi; s->psi = s->psi_next; s->psi_next = t;
+    s->led_pull += or_solute_mass(s) - m_before;
     return or_check_window(s, s->k.i_left_end);
 }
 
@@ -1051,4 +1080,29 @@ int or_common_tangent(const rs_params* p, double T, double* cl, double* cs) {
     *cl = a;
     *cs = b;
     return 1;
+}
+
+/* solute ledger (the bookkeeping rs_solute_ledger keeps on the GPU):
+ * out[0] mass, [1] mass at the reset, [2] summed pull-back change,
+ * [3]/[4] gains through the x / y boundary faces, [5] clamped
+ * cells, [6] unaccounted change */
+int or_solute_ledger(or_state* s, double* out7, int reset) {
+    const int Ny = s->p.Ny, Nx = s->p.Nx;
+    double X = 0.0, Y = 0.0;
+    int nc = 0;
+    for (int j = 1; j < Ny - 1; j++) {
+        X += s->led[j] + s->led[Ny + j];
+        nc += s->led_clamp[j];
+    }
+    for (int i = 1; i < Nx - 1; i++) Y += s->led[2 * Ny + i] + s->led[2 * Ny + Nx + i];
+    const double m = or_solute_mass(s);
+    out7[0] = m;
+    out7[1] = s->led_m0;
+    out7[2] = s->led_pull;
+    out7[3] = X;
+    out7[4] = Y;
+    out7[5] = nc;
+    out7[6] = ((m - s->led_m0) - s->led_pull) - (X + Y);
+    if (reset) or_ledger_reset(s);
+    return RS_OK;
 }

@@ -103,6 +103,11 @@ uint64_t or_splitmix64(uint64_t* state);
 
 int or_set_threads(int n);
 
+/* solute ledger: out7 = mass, mass at the reset, summed pull-back change,
+ * x and y boundary face gains, clamped cells, unaccounted change;
+ * reset != 0 restarts it at the current fields */
+int or_solute_ledger(or_state* s, double* out7, int reset);
+
 #ifdef __cplusplus
 }
 #endif

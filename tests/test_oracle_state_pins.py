@@ -315,3 +315,61 @@ def test_frozen_tip_temperature_is_the_imposed_gradient(rs, oracle_mod):
     assert b["T_tip"] - a["T_tip"] == pytest.approx(dT, rel=1e-9)
     assert a["T_tip"] == pytest.approx(p.Tmelt + p.deltaT0 * a["delta"], rel=1e-14)
     st.close()
+
+
+# ------------------------------------------------------------ solute ledger
+def test_solute_ledger_closes_on_a_coupled_run_with_pull_backs(rs, oracle_mod):
+    """Every change of the interior solute is one of: a pull-back's shift
+    (shifted-out columns, injected cinf), the flux through the x or y
+    boundary faces (the ghosts copy c and psi from their images but see
+    their own T), or the clamp; the remainder is the rounding of the updates
+    (< 1e-13 of the mass). With coupled T the y faces leak (the coarse T
+    varies along y); the x and y terms and the pull-backs are all non-zero."""
+    p = rs.config_params(256, True)
+    st = oracle_mod.State(p)
+    st.init_config(7)
+    k = st.constants()
+    k.step_pull_back = 150
+    st.set_constants(k)
+    st.step(600)
+    L = st.solute_ledger()
+    assert L["clamped"] == 0
+    assert abs(L["unaccounted"]) < 1e-13 * L["mass"]
+    assert abs(L["pull_back"]) > 1.0 and abs(L["edge_x"]) > 1e-4 and abs(L["edge_y"]) > 1e-2
+    L2 = st.solute_ledger(reset=True)
+    assert L2 == L
+    L3 = st.solute_ledger()
+    assert L3["mass0"] == L["mass"] and L3["pull_back"] == 0.0 and L3["edge_y"] == 0.0
+
+
+def test_solute_ledger_frozen_T_has_no_y_leak(rs, oracle_mod):
+    """Frozen T depends on x only: the periodic y boundary faces carry equal
+    and opposite fluxes (to rounding), the x faces a net flux."""
+    p = rs.config_params(128, False)
+    st = oracle_mod.State(p)
+    st.init_config(3)
+    st.step(300)
+    L = st.solute_ledger()
+    assert abs(L["unaccounted"]) < 1e-13 * L["mass"]
+    assert abs(L["edge_y"]) < 1e-12 * abs(L["edge_x"]) + 1e-15
+    assert abs(L["edge_x"]) > 1e-4
+
+
+def test_solute_ledger_books_a_pull_back_exactly(rs, oracle_mod):
+    """One pull-back by `off` on random fields changes the interior solute
+    by (off + 1)(Ny - 2) cinf minus the shifted-out columns 1..off and
+    Nx - 2; the ledger books exactly that."""
+    p = rs.config_params(256, True)
+    st = oracle_mod.State(p)
+    st.init_config(3)
+    rng = np.random.default_rng(8)
+    Ny, Nx = p.Ny, p.Nx
+    c0 = rng.uniform(0.5, 0.9, (Ny, Nx))
+    st.put(RS_FIELD_C, c0)
+    off = 6
+    st.pull_back(off)
+    rows = slice(1, Ny - 1)
+    want = (off + 1) * (Ny - 2) * p.cinf - c0[rows, 1:off + 1].sum() - c0[rows, Nx - 2].sum()
+    L = st.solute_ledger()
+    assert L["pull_back"] == pytest.approx(want, rel=1e-11)
+    assert abs(L["unaccounted"]) < 1e-13 * L["mass"]
