@@ -135,7 +135,7 @@ def test_batch_surrogate_matches_single_runs(rs):
     b = rs.Batch(params)
     for run, s in zip(b.runs, seeds):
         run.init_config(s)
-    rep = b.surrogate_train(epochs=4)
+    rep = b.surrogate_train()
     say("shared surrogate", rep)
     weights = b.runs[0].surrogate_get()
     b.step(150)

@@ -143,7 +143,9 @@ def test_surrogate_trained_ragged_shape(rs, oracle_runs):
     say("surrogate trained", name)
     ctx = rs.Context(_params(rs, name, RS_PREC_SURROGATE))
     ctx.init_config(3)
-    rep = ctx.surrogate_train(epochs=8)
+    # default training: 8 epochs leave the networks too few trusted bins
+    # to serve this run's cells
+    rep = ctx.surrogate_train()
     say("trusted bins", rep["trusted_frac"], "window", rep["T_lo"], rep["T_hi"])
     ctx.surrogate_stats(reset=True)
     ctx.step(STEPS)

@@ -197,7 +197,9 @@ def test_surrogate_step_vs_fp64_and_ablation(rs, oracle_mod, trained):
         say(name, "oracle drift", dmo)
         assert abs(dm) <= 1e-10 + 1e-2 * abs(dmo)
         if name.startswith("trained"):
-            assert stats["fallback_frac"] < 0.5
+            # the edge rows and columns (3% at 256^2) and little else: the
+            # window spans the 1941 K breakpoint the run's field crosses
+            assert stats["fallback_frac"] < 0.05
         else:
             assert stats["fallback_frac"] > 0.99
     for c in runs.values():
@@ -211,7 +213,9 @@ def test_surrogate_conserves_solute_exactly(rs):
     say("surrogate conservation, uniform T")
     ctx = rs.Context(p)
     ctx.init_config(3)
-    rep = ctx.surrogate_train(epochs=8)
+    # default training: at 8 epochs the network met the bound in too few
+    # bins to serve this run's cells
+    rep = ctx.surrogate_train()
     say("trusted", rep["trusted_frac"], "window", rep["T_lo"], rep["T_hi"])
     m0 = ctx.solute_mass()
     ctx.surrogate_stats(reset=True)
@@ -256,3 +260,26 @@ def test_network_in_every_row_gives_the_same_fields(rs, trained, monkeypatch):
     for mode in ("1", None):
         assert np.array_equal(out[mode][0], out["0"][0])
         assert np.array_equal(out[mode][1], out["0"][1])
+
+
+def test_window_spans_the_ti_breakpoint(rs, oracle_mod, trained):
+    """The 256^2 coupled run's field reaches above 1941 K, where the G_Ti
+    descriptions change piece (fun.cu:1628-1646): the default window spans
+    the breakpoint, and the network is trusted on both sides of it at the
+    run's compositions and within the residual bound there."""
+    ctx, rep = trained
+    say("window", rep["T_lo"], rep["T_hi"])
+    assert rep["T_lo"] < 1941.0 < rep["T_hi"]
+    T = np.repeat([1939.0, 1940.5, 1941.0, 1941.5, 1943.0], 3)
+    c = np.tile([0.88, 0.9, 0.92], 5)
+    out = ctx.surrogate_eval(T, c)
+    assert out[:, 4].all()
+    p = ctx.p
+    ex = np.array([[oracle_mod.Gl(p, t, x), oracle_mod.Gs(p, t, x), oracle_mod.Glc(p, t, x),
+                    oracle_mod.Gsc(p, t, x)] for t, x in zip(T, c)])
+    err = np.abs(out[:, :4] - ex)
+    say("largest |network - exact| (J/mol) per output", err.max(axis=0))
+    # the residual bound, 1e-4 of the scales the step divides by, is
+    # 2.05 J/mol (f_L - f_S) and 1.64 J/mol (mu) for this preset
+    assert np.all(np.abs((out[:, 0] - out[:, 1]) - (ex[:, 0] - ex[:, 1])) <= 2.5)
+    assert np.all(err[:, 2:] <= 2.0)
