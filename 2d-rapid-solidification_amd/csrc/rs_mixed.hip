@@ -448,6 +448,8 @@ template <>
 struct StepLds<4> : StepLds<1> {};
 template <>
 struct StepLds<7> : StepLds<1> {};
+template <>
+struct StepLds<8> : StepLds<1> {};
 
 template <int SURR>
 __device__ __forceinline__ void load_step_lds(StepLds<SURR>& L, const MlpInfer* mlp,
@@ -661,8 +663,8 @@ __device__ __forceinline__ TPatch tpatch_load(const StepArgsMixed& a, float* buf
 }
 
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
-// (SURR - 1) & 3 (kActBf16, kActSplit, kActFp16, kActFp16Pk); 7: FP16 with the
-// network evaluated in every row. 4 wavefronts per SIMD (<= 128
+// (SURR - 1) & 3 (kActBf16, kActSplit, kActFp16, kActFp16Pk); 7 and 8: FP16
+// and packed FP16 with the network evaluated in every row. 4 wavefronts per SIMD (<= 128
 // VGPRs, no spills) for the exact thermodynamics: at 3 (143 VGPRs) the
 // 4096^2 step takes 0.191 ms instead of 0.166 ms. The surrogate variants
 // hold the MFMA operands and run at 2 (3 spills 52 registers).
@@ -702,6 +704,7 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
                              int* variant) {
     if (variant)
         *variant = !surr                                 ? 0
+                   : a.s_fmt == kActFp16Pk && a.s_always ? 8
                    : a.s_fmt == kActFp16Pk               ? 4
                    : a.s_fmt == kActFp16 && a.s_always   ? 7
                    : a.s_fmt == kActFp16                 ? 3
@@ -711,7 +714,12 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    if (surr && a.s_fmt == kActFp16Pk) {
+    if (surr && a.s_fmt == kActFp16Pk && a.s_always) {
+        if (ident)
+            launch_mk<true, 8>(grid, block, s, a);
+        else
+            launch_mk<false, 8>(grid, block, s, a);
+    } else if (surr && a.s_fmt == kActFp16Pk) {
         if (ident)
             launch_mk<true, 4>(grid, block, s, a);
         else

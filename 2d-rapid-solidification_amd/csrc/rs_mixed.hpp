@@ -50,7 +50,7 @@ struct StepArgsMixed {
     float s_a_scale, s_a_off;  // a = T' s_a_scale + s_a_off
     float y_mean[4], y_std[4];
     int s_fmt;                 // activation format (kActBf16 ... kActFp16Pk)
-    int s_always;              // FP16 format: evaluate the network in every row (no skip test)
+    int s_always;              // FP16 formats: evaluate the network in every row (no skip test)
     // solute ledger (or null), as in StepArgsF64: boundary-face gains per
     // row (x faces) and per column (y faces)
     double* led;
@@ -62,7 +62,8 @@ struct StepArgsMixed {
 // surr: thermodynamics from the learned surrogate with exact fallback
 // *variant (optional) receives the SURR template argument of the kernel it
 // launched: 0 exact table, 1 BF16, 2 split BF16, 3 FP16 (rows without a
-// trusted cell skip the network), 4 packed FP16, 7 FP16 in every row
+// trusted cell skip the network), 4 packed FP16, 7 FP16 in every row, 8
+// packed FP16 in every row
 hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
                              int* variant = nullptr);
 hipError_t launch_T32_copy(const double* T, float* T32, int64_t n, double Tref, hipStream_t s);

@@ -126,10 +126,18 @@ double residual(const Derived& d, const double* net, const double* ex) {
     return std::max(e0, std::max(e2, e3));
 }
 
+// The packed-FP16 format rounds the activation's input, clamp-FMA and
+// product to FP16 each: its output noise varies between the validation
+// nodes by up to 24% of the residual bound (measured), so its bins are
+// judged against 0.8 of the bound, which keeps the points between the
+// nodes within it.
+constexpr double kPackedTolScale = 0.8;
+
 // network and exact targets on the (8 kTrustN + 1)^2 validation grid; bins
 // whose every grid point (edges included) meets the tolerance are trusted
 int build_trust(rs_ctx* ctx, double* max_err4, double* max_res_trusted, double* frac) {
     SurrState* s = ctx->surr;
+    const double tol = s->fmt == kActFp16Pk ? kPackedTolScale * s->tol : s->tol;
     const int G = 8 * kTrustN + 1;
     const int64_t n = (int64_t)G * G;
     std::vector<double> tc((size_t)(2 * n));
@@ -175,7 +183,7 @@ int build_trust(rs_ctx* ctx, double* max_err4, double* max_res_trusted, double* 
             double r = 0.0;
             for (int pb = 8 * ib; pb <= 8 * ib + 8; ++pb)
                 for (int pa = 8 * ia; pa <= 8 * ia + 8; ++pa) r = std::max(r, res[(size_t)pb * G + pa]);
-            if (r <= s->tol && std::isfinite(r)) {
+            if (r <= tol && std::isfinite(r)) {
                 w |= 1ULL << ia;
                 ++trusted;
                 *max_res_trusted = std::max(*max_res_trusted, r);
@@ -321,7 +329,7 @@ int rs_surrogate_defaults(rs_surrogate_opts* o) {
     o->lr_final = 3e-5;
     o->residual_tol = 1e-4;
     o->n_curve = 256;
-    o->act_format = RS_ACT_FP16;
+    o->act_format = RS_ACT_FP16_PACKED;
     return RS_OK;
 }
 

@@ -444,7 +444,7 @@ def main() -> int:
     ap.add_argument("--surrogate-weights", default=None,
                     help="npz file: install the surrogate weights saved there (profiler runs "
                          "that must not time the training), or train and save them there")
-    ap.add_argument("--act-format", default="fp16", choices=sorted(ACT_FORMATS),
+    ap.add_argument("--act-format", default="fp16-packed", choices=sorted(ACT_FORMATS),
                     help="surrogate inference format of the hidden activations")
     ap.add_argument("--ensemble-runs", type=int, default=64,
                     help="runs of the batched-ensemble measurement (0: skip it)")
@@ -482,8 +482,9 @@ def main() -> int:
             "mixed": "FP32 psi/phi and fine T, FP64 c / flux divergence / ln terms / "
                      "time accumulation, exact CALPHAD (polynomial table)",
             "surrogate": "FP32 psi/phi and fine T, FP64 c / flux divergence / ln terms / time "
-                         "accumulation, CALPHAD from the MFMA surrogate (FP16 activations) fused into the "
-                         "step (exact fallback outside the trust table)"}[a.path]
+                         "accumulation, CALPHAD from the MFMA surrogate (FP16 activations, the "
+                         "activation in packed FP16) fused into the step (exact fallback outside "
+                         "the trust table)"}[a.path]
     out = {
         "metric": METRIC,
         "value": round(head["mlups"], 2),

@@ -219,8 +219,8 @@ int rs_destroy(rs_ctx* ctx);
  * out[6] = step kernel of the last rs_step launch: -1 the FP64 kernel (or
  *          none yet), else the FP32-field kernel's thermodynamics variant
  *          (0 exact table, 1 BF16 network, 2 split-BF16, 3 FP16 with rows
- *          lacking a trusted cell skipping the network, 4 packed FP16,
- *          7 FP16 evaluated in every row),
+ *          lacking a trusted cell skipping the network, 4 packed FP16
+ *          likewise, 7 FP16 and 8 packed FP16 evaluated in every row),
  * out[7] = 1 if that launch used the identity-orientation kernel,
  * out[8] = how many launches in a row, up to the last, used that kernel. */
 int rs_context_info(rs_ctx* ctx, double* out, int32_t n);
@@ -383,7 +383,7 @@ typedef struct rs_surrogate_opts {
     double residual_tol;   /* trust bound on the residual above (1e-4)        */
     int32_t n_curve;       /* points of the liquidus/solidus tables           */
     int32_t act_format;    /* inference format of the hidden activations,
-                              RS_ACT_* (RS_ACT_FP16)                          */
+                              RS_ACT_* (RS_ACT_FP16_PACKED)                   */
 } rs_surrogate_opts;
 
 /* Hidden activations are computed in FP32 and rounded once where they enter
@@ -396,10 +396,11 @@ enum {
     RS_ACT_FP16 = 2,       /* FP16 (11 bits), FP16 MFMA for layers 2 and 3    */
     RS_ACT_FP16_PACKED = 3 /* as RS_ACT_FP16 with the activation itself in
                               packed FP16 arithmetic (input, clamp-FMA and
-                              product each rounded to FP16): ~5% faster step,
-                              but its rounding noise lets points between the
-                              validation nodes of a trusted bin exceed
-                              residual_tol by up to ~25%                      */
+                              product each rounded to FP16): 3-7% faster
+                              step. Its rounding noise varies between the
+                              validation nodes by up to ~25% of
+                              residual_tol, so its trust table is built at
+                              0.8 residual_tol (the default format)        */
 };
 
 #define RS_SURR_NPARAM 4612  /* W1[64][2] b1[64] W2[64][64] b2[64] W3[4][64] b3[4] */

@@ -10,8 +10,9 @@ Stated tolerances:
     |y_gpu - y_oracle| <= 1e-3 max|y| (activations may round the other way
     for rare elements; accumulation is FP32 vs float64)
   trusted outputs vs exact CALPHAD: residual <= residual_tol (1e-4, the
-    nondimensional bound the trust table enforces) + 2e-5 margin for points
-    between the validation grid nodes
+    nondimensional bound the trust table enforces; the default packed-FP16
+    format judges its bins against 0.8 of it, so points between the
+    validation grid nodes stay within it)
   surrogate step (256^2, 2000 steps): psi L_inf <= 2e-2, c L_inf <= 2e-3
     relative, tip velocity within 1% of the FP64 oracle
   solute mass: exact interior conservation (drift <= 1e-13 with uniform T,
@@ -138,7 +139,9 @@ def test_training_reduces_error_and_trust_holds(rs, oracle_mod, trained):
     say("trusted", int(tr.sum()), "of", n, "max residual trusted", res[tr].max(),
         "untrusted median", np.median(res[~tr]) if (~tr).any() else 0.0, "dt", k.dt)
     assert tr.sum() > 0.3 * n
-    assert res[tr].max() <= 1e-4 + 2e-5
+    # the default (packed FP16) format judges its bins against 0.8 of the
+    # bound, which keeps the points between the validation nodes within it
+    assert res[tr].max() <= 1e-4
 
 
 def _tip_pos(ctx):
@@ -154,14 +157,14 @@ def test_surrogate_step_vs_fp64_and_ablation(rs, oracle_mod, trained):
     st = oracle_mod.State(pd)
     st.init_config(7)
     ps = rs.config_params(256, True, precision=RS_PREC_SURROGATE)
-    assert meta[13] == RS_ACT_FP16  # the default inference format
+    assert meta[13] == RS_ACT_FP16_PACKED  # the default inference format
     meta_split = meta.copy()
     meta_split[13] = RS_ACT_BF16_SPLIT
-    meta_packed = meta.copy()
-    meta_packed[13] = RS_ACT_FP16_PACKED
+    meta_fp16 = meta.copy()
+    meta_fp16[13] = RS_ACT_FP16
     runs = {}
     for name, weights, m in (("trained", P, meta), ("trained-split", P, meta_split),
-                             ("trained-packed", P, meta_packed),
+                             ("trained-fp16", P, meta_fp16),
                              ("untrained", mlp.init_params(3), meta)):
         ctx = rs.Context(ps)
         ctx.init_config(7)
