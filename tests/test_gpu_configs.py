@@ -107,6 +107,34 @@ def test_surrogate_4096_long_horizon_vs_fp64_path(rs):
     assert fb < 0.1
 
 
+def test_precision_4096_vs_fp64_path(rs):
+    """configs[2] (FP32 psi/phi and T, FP64 c) and configs[3] (the surrogate
+    path) at 4096^2 beside the plain FP64 path over 6 000 steps, compared at
+    3 000 and 6 000 by the precision report (rsolid.report; its 50 000-step
+    run is the profiles' precision_4096.md): per-field errors, tip and front
+    velocity over each interval, and each path's solute ledger. Bounds: FP32
+    path psi 1e-5 absolute, c 1e-6 and T 1e-7 relative, front velocity
+    1e-5, drift within 1e-10 of the FP64 path's; surrogate path psi 2e-2, c
+    2e-3, front velocity 1e-2; every path's accounted drift (the rest of its
+    ledger: its scheme's conservation error) within 1e-12, and the drift
+    difference equal to the ledger terms' difference (pull-backs, boundary
+    faces, rest) to 1e-13."""
+    from rsolid.report import run
+    res = run(4096, 6000, 3000, ["mixed", "surrogate"], seed=5, log=lambda m: say(m[:220]))
+    for r in res["rows"]:
+        assert abs(r["unacc"]) <= 1e-12 and abs(r["unacc_fp64"]) <= 1e-12, r
+        parts = r["d_pull"] + r["d_edge"] + (r["unacc"] - r["unacc_fp64"])
+        assert abs(r["drift_diff"] - parts) <= 1e-13, r
+        if r["path"] == "mixed":
+            assert r["psi_linf"] <= 1e-5 and r["c_linf"] <= 1e-6 and r["T_linf"] <= 1e-7, r
+            assert r["front_vel_err"] <= 1e-5, r
+            assert abs(r["drift_diff"]) <= 1e-10, r
+        else:
+            assert r["psi_linf"] <= 2e-2 and r["c_linf"] <= 2e-3, r
+            assert r["front_vel_err"] <= 1e-2, r
+            assert r["fallback_frac"] < 0.1, r
+
+
 @pytest.fixture(scope="module")
 def ensemble_oracle(rs, oracle_mod):
     """The bench's seeded 64-run ensemble; for its 256^2 and 512^2 runs the
