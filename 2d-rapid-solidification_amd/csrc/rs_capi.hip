@@ -112,17 +112,22 @@ void free_all(rs_ctx* ctx) {
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
 }
 
-int ev_record(rs_ctx* ctx, int which) {
+// the next (start, stop) pair of timing series `which`, written by the
+// dispatches of the launches it is handed to; left null once the pool is full
+int ev_pair(rs_ctx* ctx, int which, LaunchEv* out) {
     std::vector<hipEvent_t>& v = ctx->ev[which];
-    const int need = ctx->nev[which] + 1;
+    const int need = ctx->nev[which] + 2;
     if ((int)v.size() < need) {
         if (v.size() >= 8192) return RS_OK;  // pool full: stop sampling
-        hipEvent_t e;
-        HIPCHK(ctx, hipEventCreate(&e));
-        v.push_back(e);
+        while ((int)v.size() < need) {
+            hipEvent_t e;
+            HIPCHK(ctx, hipEventCreate(&e));
+            v.push_back(e);
+        }
     }
-    HIPCHK(ctx, hipEventRecord(v[ctx->nev[which]], ctx->stream));
-    ctx->nev[which]++;
+    out->start = v[ctx->nev[which]];
+    out->stop = v[ctx->nev[which] + 1];
+    ctx->nev[which] += 2;
     return RS_OK;
 }
 
@@ -711,12 +716,18 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             rc = do_pull_back(ctx, it - p.i_tip_target);
             if (rc) return rc;
         }
-        // event pairs around the kernels of every timing-th step
+        // every timing-th step: the dispatches of its step kernel (with the
+        // antitrapping pass) and of its coarse-T kernels record their times
         const bool timed = ctx->timing > 0 && ctx->timing_ctr++ % ctx->timing == 0;
+        LaunchEv ev_step, ev_T;
         if (timed) {
-            int rc = ev_record(ctx, 0);
+            int rc = ev_pair(ctx, 0, &ev_step);
+            if (!rc && p.ifTFC) rc = ev_pair(ctx, 1, &ev_T);
             if (rc) return rc;
         }
+        const bool at_on = p.at_coef != 0.0;
+        const LaunchEv ev_k{ev_step.start, at_on ? nullptr : ev_step.stop};
+        const LaunchEv ev_at{nullptr, ev_step.stop};
         if (ctx->mixed) {
             StepArgsMixed a = step_args_mixed(ctx);
             if (ctx->surrogate) {
@@ -733,45 +744,35 @@ int rs_step(rs_ctx* ctx, int64_t nsteps) {
             const bool ident = o[0] == 1.0 && o[1] == 0.0 && o[3] == 0.0 && o[4] == 1.0 &&
                                o[6] == 0.0 && o[7] == 0.0;
             int v = 0;
-            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->surrogate, ctx->stream, &v));
+            HIPCHK(ctx, launch_step_mixed(a, ctx->wpb, ident, ctx->surrogate, ctx->stream, &v, ev_k));
             note_variant(ctx, v, ident ? 1 : 0);
         } else {
             const StepArgsF64 a = step_args(ctx);
-            HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream));
+            HIPCHK(ctx, launch_step_fp64(a, ctx->wpb, ctx->stream, ev_k));
             note_variant(ctx, -1, 1);
         }
-        if (p.at_coef != 0.0) {
+        if (at_on) {
             const double vdn = p.V_D > 0.0 ? p.V_D * tau0 / (p.S * p.W0) : 0.0;
             const int c0 = ctx->cur;
             if (ctx->mixed)
                 HIPCHK(ctx, launch_antitrap_f32(ctx->psi32[c0], ctx->psi32[c0 ^ 1], ctx->c[c0],
                                                 ctx->c[c0 ^ 1], p.Nx, p.Ny, p.dx, k.dt, p.at_coef,
-                                                p.k_part, vdn, ctx->stream));
+                                                p.k_part, vdn, ctx->stream, ev_at));
             else
                 HIPCHK(ctx, launch_antitrap(ctx->psi[c0], ctx->psi[c0 ^ 1], ctx->c[c0], ctx->c[c0 ^ 1],
                                             p.Nx, p.Ny, p.dx, k.dt, p.at_coef, p.k_part, vdn,
-                                            ctx->stream));
-        }
-        if (timed) {
-            int rc = ev_record(ctx, 0);
-            if (rc) return rc;
+                                            ctx->stream, ev_at));
         }
         if (p.ifTFC) {
             TempArgs t = temp_args(ctx, step);
-            if (timed) {
-                int rc = ev_record(ctx, 1);
-                if (rc) return rc;
-            }
             for (int sub = 0; sub < k.ratio_dt_dtT; ++sub) {
                 t.T = ctx->T[ctx->tcur];
                 t.Tn = ctx->T[ctx->tcur ^ 1];
                 t.T32n = ctx->mixed ? ctx->T32[ctx->tcur ^ 1] : nullptr;
-                HIPCHK(ctx, launch_temperature(t, ctx->stream));
+                const LaunchEv ev{sub == 0 ? ev_T.start : nullptr,
+                                  sub == k.ratio_dt_dtT - 1 ? ev_T.stop : nullptr};
+                HIPCHK(ctx, launch_temperature(t, ctx->stream, ev));
                 ctx->tcur ^= 1;
-            }
-            if (timed) {
-                int rc = ev_record(ctx, 1);
-                if (rc) return rc;
             }
             k.i_left_end_interp = k.i_left_end;
         }

@@ -19,6 +19,8 @@
 //   * latent heat (integrate_phi, fun.cu:1494) is accumulated per column
 //     over each 10-row block band and written as one FP64 value per column
 //     and band; the coarse T kernel finishes the 10x10 block sums.
+#include <hip/hip_ext.h>
+
 #include "rs_kernels.hpp"
 #include "rs_antitrap.hpp"
 
@@ -187,12 +189,12 @@ __global__ __launch_bounds__(256) void step_fp64_kernel(StepArgsF64 a) {
     }
 }
 
-hipError_t launch_step_fp64(const StepArgsF64& a, int wpb, hipStream_t s) {
+hipError_t launch_step_fp64(const StepArgsF64& a, int wpb, hipStream_t s, LaunchEv ev) {
     if (wpb < 1) wpb = 1;
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
-    hipLaunchKernelGGL(step_fp64_kernel, grid, block, 0, s, a);
+    hipExtLaunchKernelGGL(step_fp64_kernel, grid, block, 0, s, ev.start, ev.stop, 0, a);
     return hipGetLastError();
 }
 
@@ -208,10 +210,10 @@ __global__ __launch_bounds__(256) void temperature_kernel(TempArgs a) {
     temperature_cell(a, iT, jT);
 }
 
-hipError_t launch_temperature(const TempArgs& a, hipStream_t s) {
+hipError_t launch_temperature(const TempArgs& a, hipStream_t s, LaunchEv ev) {
     dim3 block(64, 4);
     dim3 grid((a.NxT + 63) / 64, (a.NyT + 3) / 4);
-    hipLaunchKernelGGL(temperature_kernel, grid, block, 0, s, a);
+    hipExtLaunchKernelGGL(temperature_kernel, grid, block, 0, s, ev.start, ev.stop, 0, a);
     return hipGetLastError();
 }
 
@@ -484,25 +486,26 @@ __global__ __launch_bounds__(256) void antitrap_kernel(const T* psi, const T* ps
 
 template <typename P>
 static hipError_t launch_at(const P* psi, const P* psi_n, const double* c, double* c_n, int Nx,
-                            int Ny, const AtArgs& a, hipStream_t s) {
+                            int Ny, const AtArgs& a, hipStream_t s, LaunchEv ev) {
     const int nb = (Ny - 2 + kAtBand - 1) / kAtBand;
     dim3 block(kWave, 4);
     dim3 grid((Nx - 2 + kTileOut - 1) / kTileOut, (nb + 3) / 4);
-    hipLaunchKernelGGL(antitrap_kernel<P>, grid, block, 0, s, psi, psi_n, c, c_n, Nx, Ny, a);
+    hipExtLaunchKernelGGL(antitrap_kernel<P>, grid, block, 0, s, ev.start, ev.stop, 0, psi, psi_n, c,
+                          c_n, Nx, Ny, a);
     return hipGetLastError();
 }
 
 
 hipError_t launch_antitrap(const double* psi, const double* psi_n, const double* c, double* c_n,
                            int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
-                           hipStream_t s) {
-    return launch_at(psi, psi_n, c, c_n, Nx, Ny, at_args(dx, dt, at, k, vdn), s);
+                           hipStream_t s, LaunchEv ev) {
+    return launch_at(psi, psi_n, c, c_n, Nx, Ny, at_args(dx, dt, at, k, vdn), s, ev);
 }
 
 hipError_t launch_antitrap_f32(const float* psi, const float* psi_n, const double* c, double* c_n,
                                int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
-                               hipStream_t s) {
-    return launch_at(psi, psi_n, c, c_n, Nx, Ny, at_args(dx, dt, at, k, vdn), s);
+                               hipStream_t s, LaunchEv ev) {
+    return launch_at(psi, psi_n, c, c_n, Nx, Ny, at_args(dx, dt, at, k, vdn), s, ev);
 }
 
 }  // namespace rs

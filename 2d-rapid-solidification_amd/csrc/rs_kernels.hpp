@@ -137,8 +137,16 @@ __device__ __forceinline__ void temperature_cell(const TempArgs& a, int iT, int 
     }
 }
 
-hipError_t launch_step_fp64(const StepArgsF64& a, int waves_per_block, hipStream_t s);
-hipError_t launch_temperature(const TempArgs& a, hipStream_t s);
+// Timing of a launch by its own dispatch: hipExtLaunchKernel writes the
+// kernel's start and end times into these events (null: untimed), so the
+// interval is the kernel alone, with no marker packets around it.
+struct LaunchEv {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+hipError_t launch_step_fp64(const StepArgsF64& a, int waves_per_block, hipStream_t s,
+                            LaunchEv ev = {});
+hipError_t launch_temperature(const TempArgs& a, hipStream_t s, LaunchEv ev = {});
 hipError_t launch_init_T(double* T0, double* T1, int NxT_all, int NyT_all, double Tcold, int S,
                          double W0, double dx, int r, double GG, hipStream_t s);
 hipError_t launch_init_fields(double* psi, double* c, const double* x0, int Nx, int Ny,
@@ -170,10 +178,10 @@ hipError_t launch_tip_detail_f32(const float* psi, int Nx, int Ny, double pin, i
 // or FP32); vdn = V_D in W/tau0 units (0: no trapping factor)
 hipError_t launch_antitrap(const double* psi, const double* psi_n, const double* c, double* c_n,
                            int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
-                           hipStream_t s);
+                           hipStream_t s, LaunchEv ev = {});
 hipError_t launch_antitrap_f32(const float* psi, const float* psi_n, const double* c, double* c_n,
                                int Nx, int Ny, double dx, double dt, double at, double k, double vdn,
-                               hipStream_t s);
+                               hipStream_t s, LaunchEv ev = {});
 
 // host-side helpers that fill the temperature source from the run state
 TempSrc make_tempsrc(const rs_params& p, const rs_constants& k, const double* Tcoarse, int ile);

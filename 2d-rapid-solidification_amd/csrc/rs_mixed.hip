@@ -23,6 +23,7 @@
 // The file is compiled without floating-point contraction: every FMA is
 // written out, so that the prologue of a band and the loop body of the band
 // above it produce bitwise-identical face fluxes.
+#include <hip/hip_ext.h>
 #include <math.h>
 
 #include "rs_mixed.hpp"
@@ -693,15 +694,17 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
 }
 
 template <bool IDENT, int SURR>
-static void launch_mk(dim3 grid, dim3 block, hipStream_t s, const StepArgsMixed& a) {
+static void launch_mk(dim3 grid, dim3 block, hipStream_t s, const StepArgsMixed& a, LaunchEv ev) {
     if (a.led)
-        hipLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, true>), grid, block, 0, s, a);
+        hipExtLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, true>), grid, block, 0, s, ev.start,
+                              ev.stop, 0, a);
     else
-        hipLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, false>), grid, block, 0, s, a);
+        hipExtLaunchKernelGGL((step_mixed_kernel<IDENT, SURR, false>), grid, block, 0, s, ev.start,
+                              ev.stop, 0, a);
 }
 
 hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
-                             int* variant) {
+                             int* variant, LaunchEv ev) {
     if (variant)
         *variant = !surr                                 ? 0
                    : a.s_fmt == kActFp16Pk && a.s_always ? 8
@@ -716,39 +719,39 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
     if (surr && a.s_fmt == kActFp16Pk && a.s_always) {
         if (ident)
-            launch_mk<true, 8>(grid, block, s, a);
+            launch_mk<true, 8>(grid, block, s, a, ev);
         else
-            launch_mk<false, 8>(grid, block, s, a);
+            launch_mk<false, 8>(grid, block, s, a, ev);
     } else if (surr && a.s_fmt == kActFp16Pk) {
         if (ident)
-            launch_mk<true, 4>(grid, block, s, a);
+            launch_mk<true, 4>(grid, block, s, a, ev);
         else
-            launch_mk<false, 4>(grid, block, s, a);
+            launch_mk<false, 4>(grid, block, s, a, ev);
     } else if (surr && a.s_fmt == kActFp16 && a.s_always) {
         if (ident)
-            launch_mk<true, 7>(grid, block, s, a);
+            launch_mk<true, 7>(grid, block, s, a, ev);
         else
-            launch_mk<false, 7>(grid, block, s, a);
+            launch_mk<false, 7>(grid, block, s, a, ev);
     } else if (surr && a.s_fmt == kActFp16) {
         if (ident)
-            launch_mk<true, 3>(grid, block, s, a);
+            launch_mk<true, 3>(grid, block, s, a, ev);
         else
-            launch_mk<false, 3>(grid, block, s, a);
+            launch_mk<false, 3>(grid, block, s, a, ev);
     } else if (surr && a.s_fmt == kActSplit) {
         if (ident)
-            launch_mk<true, 2>(grid, block, s, a);
+            launch_mk<true, 2>(grid, block, s, a, ev);
         else
-            launch_mk<false, 2>(grid, block, s, a);
+            launch_mk<false, 2>(grid, block, s, a, ev);
     } else if (surr) {
         if (ident)
-            launch_mk<true, 1>(grid, block, s, a);
+            launch_mk<true, 1>(grid, block, s, a, ev);
         else
-            launch_mk<false, 1>(grid, block, s, a);
+            launch_mk<false, 1>(grid, block, s, a, ev);
     } else {
         if (ident)
-            launch_mk<true, 0>(grid, block, s, a);
+            launch_mk<true, 0>(grid, block, s, a, ev);
         else
-            launch_mk<false, 0>(grid, block, s, a);
+            launch_mk<false, 0>(grid, block, s, a, ev);
     }
     return hipGetLastError();
 }

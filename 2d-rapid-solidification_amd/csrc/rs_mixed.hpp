@@ -65,7 +65,7 @@ struct StepArgsMixed {
 // trusted cell skip the network), 4 packed FP16, 7 FP16 in every row, 8
 // packed FP16 in every row
 hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool surr, hipStream_t s,
-                             int* variant = nullptr);
+                             int* variant = nullptr, LaunchEv ev = {});
 hipError_t launch_T32_copy(const double* T, float* T32, int64_t n, double Tref, hipStream_t s);
 hipError_t launch_init_fields_f32(float* psi, double* c, const double* x0, int Nx, int Ny,
                                   double dx, int mode, double cinf, double c_liq, double c_sol,

@@ -28,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "../../include/rs_peritectic.h"
@@ -716,27 +717,26 @@ int tp_step(tp_ctx* ctx, int64_t nsteps) {
     const bool mob23 = ctx->p.Da != 0.0 || ctx->p.Db != 0.0;
     for (int64_t q = 0; q < nsteps; ++q) {
         const StepArgs a = step_args(ctx);
+        // timed steps: the kernel's dispatch writes its start and end times
+        // into an event pair (hipExtLaunchKernel), no marker packets
         const bool timed = ctx->nev >= 0 && ctx->nev + 2 <= 4096;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timed) {
-            if ((int)ctx->ev.size() < ctx->nev + 2) {
-                hipEvent_t e0, e1;
-                TCHK(ctx, hipEventCreate(&e0));
-                TCHK(ctx, hipEventCreate(&e1));
-                ctx->ev.push_back(e0);
-                ctx->ev.push_back(e1);
+            while ((int)ctx->ev.size() < ctx->nev + 2) {
+                hipEvent_t e;
+                TCHK(ctx, hipEventCreate(&e));
+                ctx->ev.push_back(e);
             }
-            TCHK(ctx, hipEventRecord(ctx->ev[ctx->nev], ctx->stream));
+            e0 = ctx->ev[ctx->nev];
+            e1 = ctx->ev[ctx->nev + 1];
         }
         const dim3 grid(a.ntx * a.nty * a.nseg), block(kBX, kBY);
         if (mob23)
-            hipLaunchKernelGGL(step_kernel<true>, grid, block, 0, ctx->stream, a);
+            hipExtLaunchKernelGGL(step_kernel<true>, grid, block, 0, ctx->stream, e0, e1, 0, a);
         else
-            hipLaunchKernelGGL(step_kernel<false>, grid, block, 0, ctx->stream, a);
+            hipExtLaunchKernelGGL(step_kernel<false>, grid, block, 0, ctx->stream, e0, e1, 0, a);
         TCHK(ctx, hipGetLastError());
-        if (timed) {
-            TCHK(ctx, hipEventRecord(ctx->ev[ctx->nev + 1], ctx->stream));
-            ctx->nev += 2;
-        }
+        if (timed) ctx->nev += 2;
         ctx->cur ^= 1;
         ++ctx->step;
     }
