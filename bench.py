@@ -17,7 +17,8 @@ between steps 2000 and 6000 of an untimed horizon that every path runs
 before its warm-up with the plain FP64 path's. The surrogate is trained on
 the GPU before that horizon (outside the timed region; its time is
 reported). The kernel named in `roofline` is the one the library reports it
-launched, and HIP events bracket every timed launch of it.
+launched, and every timed launch of it records its start and end in a pair of
+HIP events through its own dispatch (hipExtLaunchKernel).
 `ensemble` adds the batched-ensemble measurement (64 runs of uneven sizes
 and undercoolings, one persistent launch per step, HIP-graph replay,
 autotuned tiles; see run_ensemble). Prints ONE JSON line.
@@ -49,7 +50,7 @@ BYTES_PER_CELL = {"fp64": 2 * 8 + 2 * 8,        # psi f64, c f64
 # algorithmic flops of one surrogate evaluation (2->64->64->4 MLP, one per
 # cell): 2 (2*64 + 64*64 + 64*4)
 FLOPS_PER_CELL_MLP = 2 * (2 * 64 + 64 * 64 + 64 * 4)
-# HIP-event pairs bracket the kernels of every timed step, so the kernel
+# HIP-event pairs time the kernels of every timed step, so the kernel
 # average covers exactly the timed launches
 TIMING_EVERY = 1
 # untimed horizon before the warm-up, the same for every path and every
