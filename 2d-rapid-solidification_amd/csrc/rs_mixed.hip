@@ -26,6 +26,8 @@
 #include <hip/hip_ext.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "rs_mixed.hpp"
 #include "rs_sgte_poly.inc"
 
@@ -663,6 +665,26 @@ __device__ __forceinline__ TPatch tpatch_load(const StepArgsMixed& a, float* buf
     return t;
 }
 
+// Dispatch order of the single-run exact-table kernel. The dispatcher hands
+// workgroup w to XCD w % kXcds; the band groups are split into kXcds strips
+// of consecutive rows, one per XCD (neighbouring tiles share an L2), and a
+// strip's (column tile, band group) items go out tile by tile, starting at
+// the side of the grid the front is held at (a.tiles_desc: from the last
+// tile down). The launch ends with the last items dispatched: at 4096^2 the
+// tiles of the interface and of the solute boundary layer ahead of it take
+// up to 1.8x the median tile's time, and row by row (the last band group of
+// a strip holds them too) the step took 151 us against 134-135 us tile by
+// tile. Every item of a strip appears once, so the order changes no result.
+__device__ __forceinline__ bool strip_item(const StepArgsMixed& a, int ntx, int nbg, int w,
+                                           int& tile, int& bg) {
+    const int xcd = w % kXcds, k = w / kXcds;
+    const int bg0 = xcd * nbg / kXcds, nb = (xcd + 1) * nbg / kXcds - bg0;
+    if (nb == 0 || k >= nb * ntx) return false;
+    bg = bg0 + k % nb;
+    tile = a.tiles_desc ? ntx - 1 - k / nb : k / nb;
+    return true;
+}
+
 // SURR: 0 exact thermodynamics, else the surrogate with activation format
 // (SURR - 1) & 3 (kActBf16, kActSplit, kActFp16, kActFp16Pk); 7 and 8: FP16
 // and packed FP16 with the network evaluated in every row. 4 wavefronts per SIMD (<= 128
@@ -676,18 +698,26 @@ __global__ __launch_bounds__(256, SURR ? 2 : 4) void step_mixed_kernel(StepArgsM
     // kernel, whose row is short, measured no gain and reads the global grid
     __shared__ float tpatch[SURR != 0 ? kTPatchN : 1];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x, nthr = blockDim.x * blockDim.y;
-    load_step_lds<SURR>(L, a.mlp, a.trust, tid, nthr);
-    // logical workgroup (column tile, band group) in XCD order
-    const int ntx = gridDim.x;
-    const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
-    TPatch tp{};
-    if constexpr (SURR != 0)
-        tp = tpatch_load(a, tpatch, g % ntx, (g / ntx) * blockDim.y, blockDim.y, tid, nthr);
-    __syncthreads();
-    unsigned nfb = 0, nall = 0;
-    step_tile<IDENT, SURR, SURR == 0, LED>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
+    if constexpr (SURR == 0) {  // 1D grid in strip order (launch_step_mixed)
+        const int ntx = (a.Nx - 2 + kTileOut - 1) / kTileOut;
+        const int nbg = (a.nbands + (int)blockDim.y - 1) / (int)blockDim.y;
+        int tile, bg;
+        if (!strip_item(a, ntx, nbg, blockIdx.x, tile, bg)) return;  // whole workgroup
+        load_step_lds<SURR>(L, a.mlp, a.trust, tid, nthr);
+        __syncthreads();
+        unsigned nfb = 0, nall = 0;
+        step_tile<IDENT, SURR, true, LED>(a, L.lntab, surr_lds<SURR>(L), tile,
+                                          bg * blockDim.y + threadIdx.y, nfb, nall, TPatch{});
+    } else {  // 2D grid: (column tile, band group) in XCD order
+        load_step_lds<SURR>(L, a.mlp, a.trust, tid, nthr);
+        const int ntx = gridDim.x;
+        const int g = xcd_order(blockIdx.y * ntx + blockIdx.x, ntx * gridDim.y);
+        const TPatch tp =
+            tpatch_load(a, tpatch, g % ntx, (g / ntx) * blockDim.y, blockDim.y, tid, nthr);
+        __syncthreads();
+        unsigned nfb = 0, nall = 0;
+        step_tile<IDENT, SURR, false, LED>(a, L.lntab, surr_lds<SURR>(L), g % ntx,
                                            (g / ntx) * blockDim.y + threadIdx.y, nfb, nall, tp);
-    if constexpr (SURR != 0) {
         if (threadIdx.x == 0 && nall > 0)
             add_counts(a.fb_count + (size_t)(g % kCountSlots) * kCountStride, nfb, nall);
     }
@@ -717,6 +747,12 @@ hipError_t launch_step_mixed(const StepArgsMixed& a, int wpb, bool ident, bool s
     if (wpb > 4) wpb = 4;
     dim3 block(kWave, wpb);
     dim3 grid((a.Nx - 2 + kTileOut - 1) / kTileOut, (a.nbands + wpb - 1) / wpb);
+    if (!surr) {  // strip order: kXcds strips of band groups, the longest sets the grid
+        int mx = 0;
+        for (int x = 0; x < kXcds; ++x)
+            mx = std::max(mx, ((x + 1) * (int)grid.y / kXcds - x * (int)grid.y / kXcds) * (int)grid.x);
+        grid = dim3(kXcds * mx, 1);
+    }
     if (surr && a.s_fmt == kActFp16Pk && a.s_always) {
         if (ident)
             launch_mk<true, 8>(grid, block, s, a, ev);

@@ -54,6 +54,9 @@ struct StepArgsMixed {
     // solute ledger (or null), as in StepArgsF64: boundary-face gains per
     // row (x faces) and per column (y faces)
     double* led;
+    // dispatch order of the single-run exact-table kernel: column tiles from
+    // the last down (the front is held in the upper half of the grid)
+    int tiles_desc;
 };
 
 // ident: the crystal orientation is the identity (the reference's only
