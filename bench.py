@@ -432,6 +432,7 @@ def _tip_x(ctx) -> float:
 
 
 def main() -> int:
+    global TV_FROM, TV_TO
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
@@ -457,6 +458,9 @@ def main() -> int:
     ap.add_argument("--peritectic", action="store_true",
                     help="time only the 3D peritectic Ti-Ag slab (512 x 512 x 128) and print "
                          "its JSON object")
+    ap.add_argument("--horizon", type=int, default=TV_TO,
+                    help="untimed steps before the warm-up (tip velocity from step "
+                         "min(2000, horizon/3) to horizon); counter runs shorten it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run a plain FP64 copy of grid^2 doubles "
                          "(byte-counter calibration for rocprofv3 --pmc)")
@@ -465,6 +469,12 @@ def main() -> int:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the y-band decomposition is a design note, not code).", file=sys.stderr)
         return 2
+    if a.horizon != TV_TO:
+        if a.horizon < 2:
+            print("--horizon must be at least 2", file=sys.stderr)
+            return 2
+        TV_TO = a.horizon
+        TV_FROM = min(TV_FROM, a.horizon // 3)
 
     if a.peritectic:
         print(json.dumps(run_peritectic(a.steps, a.warmup, 0 if a.no_cpu_baseline else 30,
