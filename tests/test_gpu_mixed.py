@@ -155,6 +155,42 @@ def test_mixed_full_size_determinism(rs):
     assert np.isfinite(outs[0][0]).all() and np.isfinite(outs[0][1]).all()
 
 
+@pytest.mark.parametrize("preset", ["config512", "reference768x272"])
+def test_dispatch_order_and_band_height_change_no_bit(rs, monkeypatch, preset):
+    """The FP32 exact-table kernel dispatches its (column tile, band group)
+    items tile by tile from the side the front is held at (config preset:
+    from the first tile up; reference preset, front at Nx - 268: from the
+    last tile down) in 20-row bands. Every item is computed once whatever
+    the order and the band height, so the fields equal, bit for bit, those
+    of 40-row bands in workgroups of one wavefront."""
+    if preset == "config512":
+        p = rs.config_params(512, True, precision=RS_PREC_MIXED)
+    else:
+        p = rs.reference_params(768, 272)
+        p.precision = RS_PREC_MIXED
+    say(preset, "i_tip_target", p.i_tip_target, "Nx", p.Nx)
+    outs = []
+    for env in ({}, {"RS_BAND_H": "40", "RS_WPB": "1"}):
+        for k in ("RS_BAND_H", "RS_WPB"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ctx = rs.Context(p)
+        say(" ", env, "band_h", ctx.info()["band_h"])
+        if preset == "config512":
+            ctx.init_config(11)
+        else:
+            ctx.init_reference()
+        ctx.step(300)
+        outs.append((ctx.download(RS_FIELD_PSI), ctx.download(RS_FIELD_C), ctx.download(RS_FIELD_T)))
+        ctx.close()
+    for k in ("RS_BAND_H", "RS_WPB"):
+        monkeypatch.delenv(k, raising=False)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.isfinite(a).all()
+        assert np.array_equal(a, b)
+
+
 def test_mixed_antitrapping_tracks_oracle(rs, oracle_mod):
     """The antitrapping / trapping extension on the mixed path (FP32 psi,
     FP64 correction of c) against the FP64 oracle, same tolerances as the
