@@ -442,11 +442,16 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     orientation(ctx->orient, ctx->p.alpha0, ctx->p.beta0, ctx->p.gamma0);
     ctx->n = (int64_t)p->Nx * p->Ny;
     ctx->nT = (int64_t)ctx->k.NxT_all * ctx->k.NyT_all;
-    // the FP32 exact-table kernel takes 20-row bands: a 4096^2 step on a
-    // developed front takes 135 us with them, 154 us with 40 rows and 137 us
-    // with 10 (its launch ends with its slowest workgroups, and shorter bands
-    // finish sooner; below 20 the two halo rows of each band cost more)
-    if (p->precision == RS_PREC_MIXED) ctx->band_h = 20;
+    // the FP32 exact-table kernel takes 20-row bands in workgroups of two
+    // wavefronts: a 4096^2 step on a developed front takes 127 us so, 134 us
+    // with four wavefronts, 130 us with one, 154 us with 40-row bands and 137
+    // us with 10 (its launch ends with its slowest workgroups, and shorter
+    // bands finish sooner; below 20 rows the two halo rows of each band cost
+    // more, and the XCD strips are whole band groups: finer groups even them)
+    if (p->precision == RS_PREC_MIXED) {
+        ctx->band_h = 20;
+        ctx->wpb = 2;
+    }
     if (const char* e = getenv("RS_BAND_H")) {
         const int v = atoi(e);
         if (v >= 10 && v <= 2000 && v % 10 == 0) ctx->band_h = v;
