@@ -359,11 +359,13 @@ hipError_t launch_mlp_grad(const float* X, const float* Yn, int n, const MlpFrag
 }
 
 // ---------------------------------------------------------------------------
-// inference over rows (tests, trust table): the fused stepper's forward
+// inference over rows (tests, trust table, rs_surrogate_time): the fused
+// stepper's forward; 4 wavefronts per SIMD for the FP16 formats (<= 128
+// registers without spills), 2 for the BF16 ones
 // ---------------------------------------------------------------------------
 template <int FMT>
-__global__ __launch_bounds__(256) void mlp_eval_kernel(const float* X, int64_t n,
-                                                       const MlpFrags* Fg, float* out) {
+__global__ __launch_bounds__(256, fp16_fmt(FMT) ? 4 : 2) void mlp_eval_kernel(
+    const float* X, int64_t n, const MlpFrags* Fg, float* out) {
     __shared__ MlpInfer F;
     copy_words(&F, fp16_fmt(FMT) ? &Fg->h : &Fg->f, threadIdx.x, blockDim.x);
     __syncthreads();
@@ -373,6 +375,20 @@ __global__ __launch_bounds__(256) void mlp_eval_kernel(const float* X, int64_t n
     const f4 y = forward<FMT>(F, input_frag(X[2 * rc], X[2 * rc + 1]), lane);
     if (r < n)
         for (int o = 0; o < 4; ++o) out[4 * r + o] = y[o];
+}
+
+__global__ void mlp_spread_inputs_kernel(float* X, int64_t n) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const double u = (double)q * 0.6180339887498949, v = (double)q * 0.7548776662466927;
+    X[2 * q] = (float)(-0.95 + 1.9 * (u - floor(u)));
+    X[2 * q + 1] = (float)(-0.95 + 1.9 * (v - floor(v)));
+}
+
+hipError_t launch_mlp_spread_inputs(float* X, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(mlp_spread_inputs_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       X, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, int fmt,

@@ -150,6 +150,9 @@ hipError_t launch_mlp_grad(const float* X, const float* Yn, int n, const MlpFrag
 // forward over n rows -> normalised outputs (n x 4)
 hipError_t launch_mlp_eval(const float* X, int64_t n, const MlpFrags* F, float* out, int fmt,
                            hipStream_t s);
+// n network inputs spread over [-0.95, 0.95]^2 (additive recurrences with
+// the golden ratios), for timing the network alone
+hipError_t launch_mlp_spread_inputs(float* X, int64_t n, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // device helpers shared by the training kernels and the fused stepper

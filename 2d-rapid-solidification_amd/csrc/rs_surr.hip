@@ -680,6 +680,36 @@ int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out) {
     return RS_OK;
 }
 
+int rs_surrogate_time(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms) {
+    if (!ctx || !avg_ms || n < 64 || n > (1LL << 26) || reps < 1) return RS_ERR_ARG;
+    if (!surr_ready(ctx)) return fail(ctx, RS_ERR_STATE, "no surrogate weights");
+    SurrState* s = ctx->surr;
+    Scratch sc;
+    float *dX, *dO;
+    HIPCHK(ctx, sc.get(&dX, 2 * (size_t)n));
+    HIPCHK(ctx, sc.get(&dO, 4 * (size_t)n));
+    HIPCHK(ctx, launch_mlp_spread_inputs(dX, n, ctx->stream));
+    HIPCHK(ctx, launch_mlp_eval(dX, n, s->F, dO, s->fmt, ctx->stream));  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, RS_ERR_HIP, "hipEventCreate");
+    }
+    hipError_t e = hipEventRecord(e0, ctx->stream);
+    for (int q = 0; q < reps && e == hipSuccess; ++q)
+        e = launch_mlp_eval(dX, n, s->F, dO, s->fmt, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(ctx, RS_ERR_HIP, hipGetErrorString(e));
+    *avg_ms = (double)ms / reps;
+    return RS_OK;
+}
+
 int rs_surrogate_stats(rs_ctx* ctx, double* out, int32_t reset) {
     if (!ctx || !out) return RS_ERR_ARG;
     if (!ctx->surr) return fail(ctx, RS_ERR_STATE, "no surrogate");

@@ -72,6 +72,7 @@ def _load() -> C.CDLL:
         "rs_surrogate_grad": (C.c_int, [vp, P(C.c_float), P(C.c_float), C.c_int32, P(C.c_float),
                                         P(C.c_double)]),
         "rs_surrogate_eval": (C.c_int, [vp, P(C.c_double), C.c_int64, P(C.c_double)]),
+        "rs_surrogate_time": (C.c_int, [vp, C.c_int64, C.c_int32, P(C.c_double)]),
         "rs_surrogate_stats": (C.c_int, [vp, P(C.c_double), C.c_int32]),
         "rs_batch_create": (C.c_int, [P(RsParams), C.c_int32, P(vp)]),
         "rs_batch_destroy": (C.c_int, [vp]),
@@ -377,6 +378,13 @@ class Context:
         self._chk(lib.rs_surrogate_eval(self._h, _dptr(tc), tc.shape[0], _dptr(out)),
                   "rs_surrogate_eval")
         return out
+
+    def surrogate_time(self, n: int, reps: int = 5) -> float:
+        """ms per evaluation of the network alone over n device-resident
+        inputs (rs_surrogate_time)."""
+        ms = C.c_double()
+        self._chk(lib.rs_surrogate_time(self._h, n, reps, C.byref(ms)), "rs_surrogate_time")
+        return ms.value
 
     def surrogate_stats(self, reset: bool = False) -> dict:
         v = (C.c_double * 5)()

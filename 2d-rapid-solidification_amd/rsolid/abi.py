@@ -109,6 +109,7 @@ EXPORTED = [
     "rs_set_finite_check",
     "rs_calibrate_copy", "rs_context_info", "rs_surrogate_defaults", "rs_surrogate_train",
     "rs_surrogate_get", "rs_surrogate_set", "rs_surrogate_grad", "rs_surrogate_eval",
+    "rs_surrogate_time",
     "rs_surrogate_stats", "rs_batch_create", "rs_batch_destroy", "rs_batch_last_error",
     "rs_batch_run", "rs_batch_step", "rs_batch_surrogate_train", "rs_batch_tune", "rs_batch_info",
     "rs_write_field_txt", "rs_write_tip_row", "rs_append_offset", "rs_append_history",

@@ -295,6 +295,9 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
     kc = ctx.constants()
     info = ctx.info()
     copy_ms = ctx.calibrate_copy(grid * grid, 5) if calibrate else None
+    # the network alone over as many cells (no stencil around it): what a
+    # fused surrogate step pays at least
+    net_ms = ctx.surrogate_time(grid * grid, 5) if path == "surrogate" else None
     ctx.close()
     cells = grid * grid
     bpc = BYTES_PER_CELL[path]
@@ -326,6 +329,7 @@ def run_path(path: str, grid: int, steps: int, warmup: int, calibrate: bool,
         "tip_advance_cells": x_to - x_from,
         "train": train,
         "fallback": fb,
+        "network_alone_ms": net_ms,
     }
 
 
@@ -542,7 +546,8 @@ def main() -> int:
                             "train_s": round(r["train"]["train_s"], 3),
                             "train_steps": int(r["train"]["steps"]),
                             "trusted_bin_frac": r["train"]["trusted_frac"],
-                            "max_residual_trusted": r["train"]["max_res_trusted"]}
+                            "max_residual_trusted": r["train"]["max_res_trusted"],
+                            "network_alone_ms": round(r["network_alone_ms"], 5)}
         if "loaded_from" in r["train"]:
             out["surrogate"]["weights_loaded_from"] = r["train"]["loaded_from"]
     if not a.only or a.antitrapping:

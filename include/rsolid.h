@@ -431,6 +431,12 @@ int rs_surrogate_grad(rs_ctx* ctx, const float* X, const float* Yn, int32_t n, f
  * mu_L, mu_S (J/mol, ideal terms added exactly) and 1.0 where the point is
  * in the domain and its bin trusted, else 0.0. */
 int rs_surrogate_eval(rs_ctx* ctx, const double* Tc, int64_t n, double* out);
+/* The network alone: the average time (ms, HIP events) of one evaluation of
+ * the current network in its inference format over n device-resident inputs
+ * (64 <= n <= 2^26) spread over the domain, reps timed launches after one
+ * warm-up, with nothing of the stencil around it -- what a fused step pays
+ * at least for n cells. Measurement only; no reference counterpart. */
+int rs_surrogate_time(rs_ctx* ctx, int64_t n, int32_t reps, double* avg_ms);
 /* Counts of the fused kernel since the last reset (out: 5 values): out[0]
  * cell evaluations by the exact routine, out[1] all evaluations, out[2]
  * their ratio, out[3] wavefront rows (62 cells) with no trusted cell, which

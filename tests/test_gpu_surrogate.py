@@ -286,3 +286,25 @@ def test_window_spans_the_ti_breakpoint(rs, oracle_mod, trained):
     # 2.05 J/mol (f_L - f_S) and 1.64 J/mol (mu) for this preset
     assert np.all(np.abs((out[:, 0] - out[:, 1]) - (ex[:, 0] - ex[:, 1])) <= 2.5)
     assert np.all(err[:, 2:] <= 2.0)
+
+
+def test_network_alone_timing(rs, trained):
+    """rs_surrogate_time: the network evaluated alone over device-resident
+    inputs takes a positive, finite time that grows with the number of
+    cells; its argument rules and the untrained state are refused."""
+    from rsolid import RsError
+    from rsolid.abi import RS_ERR_ARG, RS_ERR_STATE
+    ctx, _ = trained
+    t_small = ctx.surrogate_time(1 << 16, 3)
+    t_large = ctx.surrogate_time(1 << 22, 3)
+    say("network alone: 2^16 inputs", t_small, "ms; 2^22 inputs", t_large, "ms")
+    assert 0.0 < t_small < t_large < 1e3
+    for n, reps in ((63, 1), ((1 << 26) + 1, 1), (1 << 16, 0)):
+        with pytest.raises(RsError) as e:
+            ctx.surrogate_time(n, reps)
+        assert e.value.code == RS_ERR_ARG
+    fresh = rs.Context(rs.config_params(64, True, precision=RS_PREC_SURROGATE))
+    with pytest.raises(RsError) as e:
+        fresh.surrogate_time(1 << 16, 1)
+    assert e.value.code == RS_ERR_STATE
+    fresh.close()
