@@ -838,9 +838,12 @@ __global__ __launch_bounds__(256, SURR ? 2 : 3) void batch_step_kernel(BatchLaun
             const double dl = R.GG * (d.l_offset_Vp - R.l_off0) * R.S * R.W0 / R.deltaT0;
             a.fta_a = R.Tmelt + R.deltaT0 * (R.delta_initial - dl) + R.gstep * R.ioff - kMixedTref;
         }
+        // a run's items tile by tile (its band groups of one column tile in
+        // a row): 64-run ensemble 1.8% (FP32 table) and 3.6% (surrogate)
+        // faster than row by row
         const int local = item - L.item0[r];
-        const int tx = local % R.ntx;
-        const int bg = local / R.ntx;
+        const int tx = local / R.nbg;
+        const int bg = local % R.nbg;
         unsigned nfb = 0, nall = 0;
         // runs without a ledger take the plain tile (a.led is null for them)
         if (LED && a.led)
