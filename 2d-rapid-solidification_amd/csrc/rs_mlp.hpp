@@ -235,10 +235,11 @@ __device__ __forceinline__ f4 act4(f4 z) { return f4{act(z[0]), act(z[1]), act(z
 __device__ __forceinline__ f4 dact4(f4 z) {
     return f4{dact(z[0]), dact(z[1]), dact(z[2]), dact(z[3])};
 }
-// bias of D rows 4g..4g+3 of m-tile mt
+// bias of D rows 4g..4g+3 of m-tile mt: one 16-byte load (the bias arrays
+// are 16-byte aligned), whose constant part rides in the load's offset
+// field (two 4-byte loads took a vector add for each address)
 __device__ __forceinline__ f4 bias4(const float* b, int mt, int lane) {
-    const float* p = b + 16 * mt + 4 * (lane >> 4);
-    return f4{p[0], p[1], p[2], p[3]};
+    return *reinterpret_cast<const f4*>(b + 16 * mt + 4 * (lane >> 4));
 }
 
 // the lane's own input as a K32 fragment (a_hi, a_mid, a_lo, b_hi, b_mid,
