@@ -448,7 +448,10 @@ int rs_create(const rs_params* p, rs_ctx** out) {
     // us with 10 (its launch ends with its slowest workgroups, and shorter
     // bands finish sooner; below 20 rows the two halo rows of each band cost
     // more, and the XCD strips are whole band groups: finer groups even them)
-    if (p->precision == RS_PREC_MIXED) {
+    // the FP64 kernel (three wavefronts per SIMD) likewise: 537 us with
+    // 20-row bands in two-wavefront workgroups, 545 us with 40-row bands
+    // in four, 596 us with 60
+    if (p->precision == RS_PREC_MIXED || p->precision == RS_PREC_FP64) {
         ctx->band_h = 20;
         ctx->wpb = 2;
     }

@@ -22,7 +22,7 @@ import sys
 import numpy as np
 import pytest
 
-from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_MIXED
+from rsolid.abi import RS_FIELD_C, RS_FIELD_PSI, RS_FIELD_T, RS_PREC_FP64, RS_PREC_MIXED
 
 pytestmark = pytest.mark.gpu
 
@@ -155,16 +155,18 @@ def test_mixed_full_size_determinism(rs):
     assert np.isfinite(outs[0][0]).all() and np.isfinite(outs[0][1]).all()
 
 
-@pytest.mark.parametrize("preset", ["config512", "reference768x272"])
+@pytest.mark.parametrize("preset", ["config512", "reference768x272", "config512_fp64"])
 def test_dispatch_order_and_band_height_change_no_bit(rs, monkeypatch, preset):
     """The FP32 exact-table kernel dispatches its (column tile, band group)
     items tile by tile from the side the front is held at (config preset:
     from the first tile up; reference preset, front at Nx - 268: from the
-    last tile down) in 20-row bands. Every item is computed once whatever
-    the order and the band height, so the fields equal, bit for bit, those
-    of 40-row bands in workgroups of one wavefront."""
-    if preset == "config512":
-        p = rs.config_params(512, True, precision=RS_PREC_MIXED)
+    last tile down) in 20-row bands; the FP64 kernel takes 20-row bands as
+    well. Every item is computed once whatever the order and the band
+    height, so the fields equal, bit for bit, those of 40-row bands in
+    workgroups of one wavefront."""
+    if preset.startswith("config512"):
+        prec = RS_PREC_FP64 if preset.endswith("fp64") else RS_PREC_MIXED
+        p = rs.config_params(512, True, precision=prec)
     else:
         p = rs.reference_params(768, 272)
         p.precision = RS_PREC_MIXED
@@ -177,7 +179,7 @@ def test_dispatch_order_and_band_height_change_no_bit(rs, monkeypatch, preset):
             monkeypatch.setenv(k, v)
         ctx = rs.Context(p)
         say(" ", env, "band_h", ctx.info()["band_h"])
-        if preset == "config512":
+        if preset.startswith("config512"):
             ctx.init_config(11)
         else:
             ctx.init_reference()
