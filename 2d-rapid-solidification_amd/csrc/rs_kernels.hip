@@ -60,7 +60,7 @@ __device__ __forceinline__ Cell make_cell(const StepArgsF64& a, double psi, doub
     q.phi = fmax(psi, -1.0);
     q.S = qq<double>(a.AA, a.q_solid, psi) * c * (1.0 - c);
     const double T = temp_at(a.ts, i, j);
-    const Thermo th = thermo_fp64(a.rk, T, c);
+    const Thermo th = thermo_fp64<true>(a.rk, T, c);
     const double g = gg<double>(psi);
     q.mu = 0.5 * (1.0 + g) * th.muS + 0.5 * (1.0 - g) * th.muL;
     q.dG = th.dG;
@@ -85,6 +85,8 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
     const int Nx = a.Nx;
     // the solute ledger's x faces: does this wavefront own column 1 or Nx-2
     const bool led_wave = a.led && __ballot(own && (i == 1 || i == Nx - 2)) != 0;
+    // divisions by the run's constants as products with their reciprocals
+    const double idG = 1.0 / a.dG_den, idx2 = 1.0 / a.dx2;
 
     // rows jb-1 and jb
     Cell m = make_cell(a, a.psi[(int64_t)(jb - 1) * Nx + ic], a.c[(int64_t)(jb - 1) * Nx + ic], ic,
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
                                                  phL_p, p.phi, phR_p, as, ak);
             const double p00 = z.phi;
             double term = p00 - (p00 * p00) * p00 + an;
-            term += z.dG * dgg<double>(p00) / a.dG_den;
+            term += z.dG * dgg<double>(p00) * idG;
             double phn = p00 + a.dt * ak / (as * as) * term;  // tau = 1 (one orientation)
             // ---- psi <= 1 fix (compute_psi, fun.cu:1243-1261) ----
             double psn = phn;
@@ -139,15 +141,15 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
             const double F01 = 0.25 * (S01 + S00 + Shh + Snh) * (p.mu - f00) * a.mu_den;
             const double Fm0 = 0.25 * (S00 + Sm0 + Snh + Snn) * (muL - f00) * a.mu_den;
             const double F0m = 0.25 * (S00 + S0m + Shn + Snn) * (m.mu - f00) * a.mu_den;
-            const double dc = a.kdc * (F10 + F01 + Fm0 + F0m) / a.dx2;
+            const double dc = a.kdc * (F10 + F01 + Fm0 + F0m) * idx2;
             const double cn = fmax(z.c + dc, 1.0e-6);
             // ledger: the flux through the boundary faces (ghost side); the
             // test is wavefront-uniform, taken by the edge tiles and rows only
             if (led_wave || (a.led && (j == 1 || j == a.Ny - 2))) {
-                if (i == 1) a.led[j] += a.kdc * Fm0 / a.dx2;
-                if (i == Nx - 2) a.led[a.Ny + j] += a.kdc * F10 / a.dx2;
-                if (j == 1) a.led[2 * a.Ny + i] += a.kdc * F0m / a.dx2;
-                if (j == a.Ny - 2) a.led[2 * a.Ny + Nx + i] += a.kdc * F01 / a.dx2;
+                if (i == 1) a.led[j] += a.kdc * Fm0 * idx2;
+                if (i == Nx - 2) a.led[a.Ny + j] += a.kdc * F10 * idx2;
+                if (j == 1) a.led[2 * a.Ny + i] += a.kdc * F0m * idx2;
+                if (j == a.Ny - 2) a.led[2 * a.Ny + Nx + i] += a.kdc * F01 * idx2;
             }
 
             // ---- stores + ghost images ----

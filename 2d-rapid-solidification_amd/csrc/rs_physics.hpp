@@ -66,23 +66,33 @@ struct Unaries {
     double ti_bcc, ti_liq, nb_bcc, nb_liq;
 };
 
+// C / T as written (FAST = false), or as C times one reciprocal of T shared
+// by the four unaries (FAST = true: the FP64 step kernel, where each IEEE
+// FP64 division is a sequence of about eleven instructions)
+template <bool FAST>
+RS_HD double over_T(double C, double T, double iT) {
+    return FAST ? C * iT : C / T;
+}
+
+template <bool FAST = false>
 RS_HD Unaries sgte_unaries(double T, double lT) {
     Unaries u;
     const double t2 = T * T;
     const double t3 = t2 * T;
+    const double iT = FAST ? 1.0 / T : 0.0;
     // G_Ti_BCC fun.cu:1628-1636
     if (T >= 1155.0 && T < 1941.0)
         u.ti_bcc = 6667.385 + 105.366379 * T - 22.3771 * T * lT + 1.21707e-3 * t2 -
-                   0.84534e-6 * t3 - 2002750.0 / T;
+                   0.84534e-6 * t3 - over_T<FAST>(2002750.0, T, iT);
     else if (T >= 1941.0)
         u.ti_bcc = 26483.26 - 182.426471 * T + 19.0900905 * T * lT - 22.00832e-3 * t2 +
-                   1.228863e-6 * t3 + 1400501.0 / T;
+                   1.228863e-6 * t3 + over_T<FAST>(1400501.0, T, iT);
     else
         u.ti_bcc = NAN;
     // G_Ti_Liq fun.cu:1638-1646
     if (T >= 1300.0 && T < 1941.0)
         u.ti_liq = 369519.198 - 2554.0225 * T + 342.059267 * T * lT - 163.409355e-3 * t2 +
-                   12.457117e-6 * t3 - 67034516.0 / T;
+                   12.457117e-6 * t3 - over_T<FAST>(67034516.0, T, iT);
     else if (T >= 1941.0 && T < 4000.0)
         u.ti_liq = -19887.066 + 298.7367 * T - 46.29 * T * lT;
     else
@@ -90,7 +100,7 @@ RS_HD Unaries sgte_unaries(double T, double lT) {
     // G_Nb_BCC fun.cu:1648-1656
     if (T >= 298.15 && T < 2750.0)
         u.nb_bcc = -8519.353 + 142.045475 * T - 26.4711 * T * lT + 0.203475e-3 * t2 -
-                   0.35012e-6 * t3 + 93399.0 / T;
+                   0.35012e-6 * t3 + over_T<FAST>(93399.0, T, iT);
     else if (T >= 2750.0)
         u.nb_bcc = -37669.3 + 271.720843 * T - 41.77 * T * lT + 1528.238e29 / ((t3 * t3) * t3);
     else
@@ -98,7 +108,7 @@ RS_HD Unaries sgte_unaries(double T, double lT) {
     // G_Nb_Liq fun.cu:1658-1666
     if (T >= 298.15 && T < 2750.0)
         u.nb_liq = 21262.202 + 131.229057 * T - 26.4711 * T * lT + 0.203475e-3 * t2 -
-                   0.35012e-6 * t3 + 93399.0 / T - 306.098e-25 * (((t2 * t2) * t2) * T);
+                   0.35012e-6 * t3 + over_T<FAST>(93399.0, T, iT) - 306.098e-25 * (((t2 * t2) * t2) * T);
     else if (T >= 2750.0)
         u.nb_liq = -7499.398 + 260.756148 * T - 41.77 * T * lT;
     else
@@ -128,9 +138,14 @@ RS_HD double ideal_mix(double RR, double T, double c, double onemc, double lc, d
     return RR * T * (onemc * l1c + c * lc);
 }
 
+// FAST (the FP64 step kernel): the SGTE terms in 1/T share one reciprocal,
+// and log(c/(1-c)) is log c - log(1-c) from the two logs already taken
+// (one log and one division fewer per cell; each result within a few ulp of
+// the reference's form)
+template <bool FAST = false>
 RS_HD Thermo thermo_fp64(const RK& rk, double T, double c) {
     const double lT = log(T);
-    const Unaries u = sgte_unaries(T, lT);
+    const Unaries u = sgte_unaries<FAST>(T, lT);
     const double onemc = 1.0 - c;
     // the logs are evaluated only where the reference evaluates them
     const double lc = (onemc < 1.e-8 || c >= 1.e-8) ? log(c) : 0.0;
@@ -141,7 +156,7 @@ RS_HD Thermo thermo_fp64(const RK& rk, double T, double c) {
     const double Gl = id + onemc * u.nb_liq + c * u.ti_liq + exl;
     const double Gs = id + onemc * u.nb_bcc + c * u.ti_bcc + exs;
     double lr = 0.;
-    if (c > 1.e-8 && onemc > 1.e-8) lr = rk.RR * T * log(c / onemc);
+    if (c > 1.e-8 && onemc > 1.e-8) lr = rk.RR * T * (FAST ? lc - l1c : log(c / onemc));
     const double tl = rk.L0_liq * (1.0 - 2.0 * c) + rk.L1_liq * (1.0 - 6.0 * c + 6.0 * (c * c));
     const double ts = rk.L0_bcc * (1.0 - 2.0 * c) + rk.L1_bcc * (1.0 - 6.0 * c + 6.0 * (c * c));
     Thermo th;
@@ -204,25 +219,32 @@ struct Aniso {
     R dx;
 };
 
+// The divisions of fun.cu:1040-1075 by the grid constants 2 dx, dx^2,
+// 4 dx^2 and 2 dx^2 are products with their reciprocals (loop invariants,
+// computed once per wavefront), and the four by |grad phi| and its square
+// share one reciprocal of |grad phi|.
 template <typename R>
 RS_HD R anisotropy(const Aniso<R>& A, R pmm, R p0m, R p1m, R pm0, R p00, R p10, R pm1, R p01,
                    R p11, R& as, R& ak) {
     const R dx = A.dx;
-    const R gx = (p10 - pm0) / (R(2.0) * dx);
-    const R gy = (p01 - p0m) / (R(2.0) * dx);
+    const R i2dx = R(1.0) / (R(2.0) * dx), idx2 = R(1.0) / (dx * dx);
+    const R i4dx2 = R(1.0) / (R(4.) * dx * dx), i2dx2 = R(1.0) / (R(2.0) * dx * dx);
+    const R gx = (p10 - pm0) * i2dx;
+    const R gy = (p01 - p0m) * i2dx;
     const R ag = sqrt(gx * gx + gy * gy);
-    const R gxx = (p10 - R(2.0) * p00 + pm0) / (dx * dx);
-    const R gyy = (p01 - R(2.0) * p00 + p0m) / (dx * dx);
-    const R gxy = (p11 - pm1 - p1m + pmm) / (R(4.) * dx * dx);
+    const R gxx = (p10 - R(2.0) * p00 + pm0) * idx2;
+    const R gyy = (p01 - R(2.0) * p00 + p0m) * idx2;
+    const R gxy = (p11 - pm1 - p1m + pmm) * i4dx2;
     R lap = A.lap_w_axis * (gxx + gyy);
-    lap += A.lap_w_diag * (p11 + pm1 + p1m + pmm - R(4.0) * p00) / (R(2.0) * dx * dx);
+    lap += A.lap_w_diag * (p11 + pm1 + p1m + pmm - R(4.0) * p00) * i2dx2;
     R thx, thy, ct, st;
     if (ag > R(1.0e-15)) {
-        const R ag2 = ag * ag;
-        thx = (gx * gxy - gy * gxx) / ag2;
-        thy = (gx * gyy - gy * gxy) / ag2;
-        ct = gx / ag;
-        st = gy / ag;
+        const R iag = R(1.0) / ag;
+        const R iag2 = iag * iag;
+        thx = (gx * gxy - gy * gxx) * iag2;
+        thy = (gx * gyy - gy * gxy) * iag2;
+        ct = gx * iag;
+        st = gy * iag;
     } else {
         thx = thy = ct = st = R(0.0);
     }
