@@ -6,7 +6,10 @@
 // FP64 functions keep the operand order of the reference expressions they
 // restate (fun.cu cited per function) so that, with floating-point
 // contraction disabled, they round exactly like the reference wherever the
-// math library agrees.
+// math library agrees. The exceptions are marked: the FAST form of the
+// CALPHAD evaluation and the anisotropic operator's reciprocals, which the
+// FP64 step kernel uses to save IEEE divisions and a log per cell (each
+// result within a few ulp of the reference's form).
 #pragma once
 
 #include <hip/hip_runtime.h>

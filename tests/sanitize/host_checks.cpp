@@ -8,9 +8,10 @@
 // the tip fit (rs::precise_tip: row scans, quartic and parabola solves),
 // every file format (field text, tip rows, offset and history records,
 // checkpoints, parameters.txt write/read, Ti-Ag cross sections and break
-// points), argument rejections, and the oracle's whole-run paths (coupled
+// points), argument rejections, the oracle's whole-run paths (coupled
 // and frozen T, pull-backs of both signs, antitrapping, tip outputs, the
-// Ti-Ag restatement).
+// Ti-Ag restatement), and the FP64 step kernel's form of the CALPHAD
+// evaluation against the reference's form.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -194,12 +195,46 @@ static void peritectic(const char* dir) {
     CHECK(tp_read_break_point(tmp_path(dir, "xz.txt").c_str(), back.data(), p.Nx, p.Ny, p.Nz) != RS_OK);
 }
 
+// thermo_fp64<true> (one reciprocal of T for the SGTE terms in 1/T, and
+// log(c/(1-c)) as log c - log(1-c)) against the form the reference writes
+// (fun.cu:1628-1720) over the SGTE ranges and the composition branches: the
+// two agree to rounding (at most 6e-11 J/mol on values of 1e2-1e5 J/mol), and
+// both are NaN outside the ranges. A wrong coefficient or sign in either
+// form moves a value by 1e-3 J/mol or more.
+static void thermo_forms() {
+    rs_params p;
+    CHECK(rs_params_reference(&p, 512, 272) == RS_OK);
+    const rs::RK rk = rs::rk_of(p);
+    const double cs[] = {0.0, 1e-9, 1e-8, 2e-8, 1e-6, 0.01, 0.1, 0.3, 0.5,
+                         0.6325, 0.9, 0.999999, 1.0 - 2e-8, 1.0 - 1e-9, 1.0};
+    double worst = 0.0;
+    int n = 0;
+    for (double T = 250.0; T <= 4100.0; T += 3.7) {
+        for (double c : cs) {
+            const rs::Thermo a = rs::thermo_fp64<false>(rk, T, c);
+            const rs::Thermo b = rs::thermo_fp64<true>(rk, T, c);
+            const double va[3] = {a.dG, a.muL, a.muS}, vb[3] = {b.dG, b.muL, b.muS};
+            for (int q = 0; q < 3; ++q) {
+                CHECK(isnan(va[q]) == isnan(vb[q]));
+                if (isnan(va[q])) continue;
+                const double d = fabs(va[q] - vb[q]);
+                CHECK(d <= 1e-11 + 1e-13 * fabs(va[q]));
+                if (d > worst) worst = d;
+                ++n;
+            }
+        }
+    }
+    CHECK(n > 1000);
+    printf("thermo forms: %d values, largest difference %.3g J/mol\n", n, worst);
+}
+
 int main(int argc, char** argv) {
     const char* dir = argc > 1 ? argv[1] : ".";
     params_and_derive();
     oracle_and_tip();
     formats(dir);
     peritectic(dir);
+    thermo_forms();
     printf(g_fail ? "host checks FAILED\n" : "host checks ok\n");
     return g_fail;
 }
