@@ -26,8 +26,21 @@
 
 namespace rs {
 
-__device__ __forceinline__ double shfl_up1(double v) { return __shfl_up(v, 1, kWave); }
-__device__ __forceinline__ double shfl_dn1(double v) { return __shfl_down(v, 1, kWave); }
+// lane l <- lane l-1 / lane l+1 through DPP wavefront shifts (wave_shr:1 /
+// wave_shl:1), one VALU move per 32-bit half instead of an LDS permute; the
+// lane with no source gets 0, a halo lane whose result is not stored
+__device__ __forceinline__ int dpp_up1(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ int dpp_dn1(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ double shfl_up1(double v) {
+    return __hiloint2double(dpp_up1(__double2hiint(v)), dpp_up1(__double2loint(v)));
+}
+__device__ __forceinline__ double shfl_dn1(double v) {
+    return __hiloint2double(dpp_dn1(__double2hiint(v)), dpp_dn1(__double2loint(v)));
+}
 
 // fine temperature at (i, j) as the reference's T_fine / get_delta give it
 __device__ __forceinline__ double temp_at(const TempSrc& t, int i, int j) {
