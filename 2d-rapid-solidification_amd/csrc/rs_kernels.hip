@@ -11,7 +11,7 @@
 //   * a wavefront owns a 62-column strip and marches down a band of rows,
 //     keeping a 3-row window of phi, S = q(psi)c(1-c) and mu in registers;
 //     the i+-1 neighbours come from cross-lane shuffles, so every cell's
-//     CALPHAD evaluation (4 logs) is done once and shared with its
+//     CALPHAD evaluation (3 table logs) is done once and shared with its
 //     neighbours;
 //   * ghost cells are written by the lane that owns their source cell, so the
 //     boundary copies of compute_boundary_individual (fun.cu:1220) are
@@ -23,6 +23,7 @@
 
 #include "rs_kernels.hpp"
 #include "rs_antitrap.hpp"
+#include "rs_sgte_poly.inc"
 
 namespace rs {
 
@@ -65,25 +66,31 @@ struct Cell {
     double psi, c, phi, S, mu, dG;
 };
 
-__device__ __forceinline__ Cell make_cell(const StepArgsF64& a, double psi, double c, int i,
-                                          int j) {
+__device__ __forceinline__ Cell make_cell(const StepArgsF64& a, const double* lntab, double psi,
+                                          double c, int i, int j) {
     Cell q;
     q.psi = psi;
     q.c = c;
     q.phi = fmax(psi, -1.0);
     q.S = qq<double>(a.AA, a.q_solid, psi) * c * (1.0 - c);
     const double T = temp_at(a.ts, i, j);
-    const Thermo th = thermo_fp64<true>(a.rk, T, c);
+    const Thermo th = thermo_fp64<true>(a.rk, T, c, TabLog{lntab});
     const double g = gg<double>(psi);
     q.mu = 0.5 * (1.0 + g) * th.muS + 0.5 * (1.0 - g) * th.muL;
     q.dG = th.dG;
     return q;
 }
 
-// three wavefronts per SIMD (167 VGPRs, four 8-byte spills): 537 us for a
-// 4096^2 step on a developed front, against 598 us at two (181 VGPRs) and
-// 720 us at four (128 VGPRs, 50 registers spilled)
+// three wavefronts per SIMD: with every division and log as the reference
+// writes them, 537 us for a 4096^2 step on a developed front, against 598 us
+// at two (181 VGPRs) and 720 us at four (128 VGPRs, 50 registers spilled)
 __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
+    // the log table of the CALPHAD evaluation (ln_tab), staged in LDS
+    __shared__ double lntab[2 * RS_LN_TAB_N];
+    for (int q = threadIdx.y * blockDim.x + threadIdx.x; q < 2 * RS_LN_TAB_N;
+         q += blockDim.x * blockDim.y)
+        lntab[q] = kLnTab[q];
+    __syncthreads();
     const int lane = threadIdx.x;
     // logical workgroup (column tile, band group) in XCD order
     const int ntx = gridDim.x;
@@ -102,9 +109,9 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
     const double idG = 1.0 / a.dG_den, idx2 = 1.0 / a.dx2;
 
     // rows jb-1 and jb
-    Cell m = make_cell(a, a.psi[(int64_t)(jb - 1) * Nx + ic], a.c[(int64_t)(jb - 1) * Nx + ic], ic,
-                       jb - 1);
-    Cell z = make_cell(a, a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic], ic, jb);
+    Cell m = make_cell(a, lntab, a.psi[(int64_t)(jb - 1) * Nx + ic],
+                       a.c[(int64_t)(jb - 1) * Nx + ic], ic, jb - 1);
+    Cell z = make_cell(a, lntab, a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic], ic, jb);
     double phL_m = shfl_up1(m.phi), phR_m = shfl_dn1(m.phi);
     double SL_m = shfl_up1(m.S), SR_m = shfl_dn1(m.S);
     double phL_z = shfl_up1(z.phi), phR_z = shfl_dn1(z.phi);
@@ -122,7 +129,7 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
             nx_psi = a.psi[(int64_t)jn * Nx + ic];
             nx_c = a.c[(int64_t)jn * Nx + ic];
         }
-        const Cell p = make_cell(a, cur_psi, cur_c, ic, j + 1);
+        const Cell p = make_cell(a, lntab, cur_psi, cur_c, ic, j + 1);
         const double phL_p = shfl_up1(p.phi), phR_p = shfl_dn1(p.phi);
         const double SL_p = shfl_up1(p.S), SR_p = shfl_dn1(p.S);
         const double muL = shfl_up1(z.mu), muR = shfl_dn1(z.mu);

@@ -52,27 +52,6 @@ __device__ __forceinline__ double shdn(double v) {
 
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// ln r for normal r > 0, FP64, from the LDS table tab[2k] = 1/F_k,
-// tab[2k+1] = ln F_k with F_k = 1 + (k + 1/2)/128 (error ~2 ulp)
-__device__ __forceinline__ double ln_tab(double r, const double* tab) {
-    int e;
-    const double m = frexp(r, &e);       // m in [0.5, 1)
-    const double m2 = m + m;              // [1, 2), exact
-    const int hi = __double2hiint(m2);
-    const int k = (hi >> 13) & 127;       // top 7 mantissa bits
-    const double F = __hiloint2double((hi & (int)0xFFFFE000) | 0x1000, 0);
-    const double d = m2 - F;              // exact, |d| <= 2^-8
-    const double u = d * tab[2 * k];
-    double q = fma(u, -1.0 / 6.0, 0.2);
-    q = fma(u, q, -0.25);
-    q = fma(u, q, 1.0 / 3.0);
-    q = fma(u, q, -0.5);
-    const double l1p = fma(u * u, q, u);  // log1p(u), |u| < 2^-8
-    const double ef = (double)(e - 1);
-    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
-    return fma(ef, ln2_hi, tab[2 * k + 1]) + fma(ef, ln2_lo, l1p);
-}
-
 // c/(1-c) with a Newton-refined reciprocal (~1 ulp)
 __device__ __forceinline__ double ratio_c(double c, double om) {
     double y = __builtin_amdgcn_rcp(om);

@@ -69,6 +69,49 @@ struct Unaries {
     double ti_bcc, ti_liq, nb_bcc, nb_liq;
 };
 
+// ln r for a normal r > 0 from a 128-entry table tab[2k] = 1/F_k,
+// tab[2k+1] = ln F_k, F_k = 1 + (k + 1/2)/128 (Tang's table method): with
+// r = 2^(e-1) m2, m2 in [1, 2), and F the table point of m2's top 7 mantissa
+// bits, ln r = (e-1) ln 2 + ln F + log1p(u), u = (m2 - F)/F, |u| < 2^-8, and
+// log1p by its degree-6 Taylor polynomial (truncation below 3e-18). The sum
+// (e-1) ln2_hi + ln F is carried with its rounding error (Fast2Sum: the first
+// term is exact and, when non-zero, at least ln 2 > |ln F|), so the result is
+// rounded once: within about half an ulp plus the table's rounding, like the
+// math library's log, in ~25 instructions against its ~90.
+RS_HD double ln_tab(double r, const double* tab) {
+    int e;
+    const double m = frexp(r, &e);  // m in [0.5, 1)
+    const double m2 = m + m;        // [1, 2), exact
+    const uint64_t bits = __builtin_bit_cast(uint64_t, m2);
+    const uint32_t hi = (uint32_t)(bits >> 32);
+    const int k = (int)((hi >> 13) & 127u);  // top 7 mantissa bits
+    const double F = __builtin_bit_cast(double, (uint64_t)((hi & 0xFFFFE000u) | 0x1000u) << 32);
+    const double d = m2 - F;  // exact, |d| <= 2^-8
+    const double u = d * tab[2 * k];
+    double q = fma(u, -1.0 / 6.0, 0.2);
+    q = fma(u, q, -0.25);
+    q = fma(u, q, 1.0 / 3.0);
+    q = fma(u, q, -0.5);
+    const double l1p = fma(u * u, q, u);  // log1p(u)
+    const double ef = (double)(e - 1);
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    const double lnF = tab[2 * k + 1];
+    const double a = ef * ln2_hi;  // exact: ln2_hi has 32 significant bits
+    const double s = a + lnF;
+    const double t = (a - s) + lnF;  // the rounding error of s, exactly
+    return s + (t + fma(ef, ln2_lo, l1p));
+}
+
+// the natural log of the CALPHAD evaluation: the math library's, or the
+// table log above (the FP64 step kernel, with the table in LDS)
+struct LibLog {
+    RS_HD double operator()(double x) const { return log(x); }
+};
+struct TabLog {
+    const double* tab;
+    RS_HD double operator()(double x) const { return ln_tab(x, tab); }
+};
+
 // C / T as written (FAST = false), or as C times one reciprocal of T shared
 // by the four unaries (FAST = true: the FP64 step kernel, where each IEEE
 // FP64 division is a sequence of about eleven instructions)
@@ -144,22 +187,23 @@ RS_HD double ideal_mix(double RR, double T, double c, double onemc, double lc, d
 // FAST (the FP64 step kernel): the SGTE terms in 1/T share one reciprocal,
 // and log(c/(1-c)) is log c - log(1-c) from the two logs already taken
 // (one log and one division fewer per cell; each result within a few ulp of
-// the reference's form)
-template <bool FAST = false>
-RS_HD Thermo thermo_fp64(const RK& rk, double T, double c) {
-    const double lT = log(T);
+// the reference's form). Ln: the log taken (LibLog, or TabLog in the step
+// kernel).
+template <bool FAST = false, typename Ln = LibLog>
+RS_HD Thermo thermo_fp64(const RK& rk, double T, double c, Ln ln = Ln{}) {
+    const double lT = ln(T);
     const Unaries u = sgte_unaries<FAST>(T, lT);
     const double onemc = 1.0 - c;
     // the logs are evaluated only where the reference evaluates them
-    const double lc = (onemc < 1.e-8 || c >= 1.e-8) ? log(c) : 0.0;
-    const double l1c = (c < 1.e-8 || onemc >= 1.e-8) ? log(onemc) : 0.0;
+    const double lc = (onemc < 1.e-8 || c >= 1.e-8) ? ln(c) : 0.0;
+    const double l1c = (c < 1.e-8 || onemc >= 1.e-8) ? ln(onemc) : 0.0;
     const double id = ideal_mix(rk.RR, T, c, onemc, lc, l1c);
     const double exl = c * onemc * (rk.L0_liq + rk.L1_liq * (1.0 - 2.0 * c));
     const double exs = c * onemc * (rk.L0_bcc + rk.L1_bcc * (1.0 - 2.0 * c));
     const double Gl = id + onemc * u.nb_liq + c * u.ti_liq + exl;
     const double Gs = id + onemc * u.nb_bcc + c * u.ti_bcc + exs;
     double lr = 0.;
-    if (c > 1.e-8 && onemc > 1.e-8) lr = rk.RR * T * (FAST ? lc - l1c : log(c / onemc));
+    if (c > 1.e-8 && onemc > 1.e-8) lr = rk.RR * T * (FAST ? lc - l1c : ln(c / onemc));
     const double tl = rk.L0_liq * (1.0 - 2.0 * c) + rk.L1_liq * (1.0 - 6.0 * c + 6.0 * (c * c));
     const double ts = rk.L0_bcc * (1.0 - 2.0 * c) + rk.L1_bcc * (1.0 - 6.0 * c + 6.0 * (c * c));
     Thermo th;

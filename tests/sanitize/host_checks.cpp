@@ -10,8 +10,9 @@
 // checkpoints, parameters.txt write/read, Ti-Ag cross sections and break
 // points), argument rejections, the oracle's whole-run paths (coupled
 // and frozen T, pull-backs of both signs, antitrapping, tip outputs, the
-// Ti-Ag restatement), and the FP64 step kernel's form of the CALPHAD
-// evaluation against the reference's form.
+// Ti-Ag restatement), the FP64 step kernel's form of the CALPHAD
+// evaluation against the reference's form, and the table log against the C
+// library's.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -228,6 +229,48 @@ static void thermo_forms() {
     printf("thermo forms: %d values, largest difference %.3g J/mol\n", n, worst);
 }
 
+// ln_tab (the table log of the FP64 step kernel and of the FP32 paths'
+// ln(c/(1-c))) against the C library's log, over the arguments the steppers
+// give it: within one ulp or 6e-17 (the rounding of the table's ln F, which
+// is what is left where the result cancels to near 0, r near 1), and equal
+// to the library's value for at least 97% of the temperatures (log T carries
+// the largest multipliers of the SGTE terms). The table is rebuilt here as
+// the build's generator writes it: tab[2k] = 1/F_k, tab[2k+1] = ln F_k.
+static void table_log() {
+    double tab[256];
+    for (int k = 0; k < 128; ++k) {
+        const double F = 1.0 + (k + 0.5) / 128.0;
+        tab[2 * k] = (double)(1.0L / (long double)F);
+        tab[2 * k + 1] = (double)logl((long double)F);
+    }
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    auto next = [&st]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return (double)(st >> 11) * 0x1p-53;
+    };
+    long n[2] = {0, 0}, same[2] = {0, 0};
+    auto one = [&](double r, int g) {
+        const double a = rs::ln_tab(r, tab), b = log(r);
+        const double ulp = nextafter(fabs(b), INFINITY) - fabs(b);
+        CHECK(fabs(a - b) <= ulp + 6e-17);
+        same[g] += a == b;
+        ++n[g];
+    };
+    for (int q = 0; q < 400000; ++q) {
+        one(exp(log(1e-8) * next()), 0);  // compositions (1e-8, 1], log-uniform
+        one(1.0 - 0.999 * next(), 0);     // their complements, near 1
+        one(250.0 + 3850.0 * next(), 1);  // temperatures
+    }
+    for (int b = -30; b <= 12; ++b)
+        for (double f : {1.0, 1.0 + 0x1p-52, 2.0 - 0x1p-52, 1.00390625, 1.99609375})
+            one(ldexp(f, b), 0);
+    CHECK(same[1] >= 0.97 * n[1]);
+    printf("table log: %ld arguments, equal to libm for %.4f of the temperatures and %.4f of the rest\n",
+           n[0] + n[1], (double)same[1] / n[1], (double)same[0] / n[0]);
+}
+
 int main(int argc, char** argv) {
     const char* dir = argc > 1 ? argv[1] : ".";
     params_and_derive();
@@ -235,6 +278,7 @@ int main(int argc, char** argv) {
     formats(dir);
     peritectic(dir);
     thermo_forms();
+    table_log();
     printf(g_fail ? "host checks FAILED\n" : "host checks ok\n");
     return g_fail;
 }
