@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
             const double p00 = z.phi;
             double term = p00 - (p00 * p00) * p00 + an;
             term += z.dG * dgg<double>(p00) * idG;
-            double phn = p00 + a.dt * ak / (as * as) * term;  // tau = 1 (one orientation)
+            double phn = p00 + a.dt * ak * rcp_nr(as * as) * term;  // tau = 1 (one orientation)
             // ---- psi <= 1 fix (compute_psi, fun.cu:1243-1261) ----
             double psn = phn;
             if (psn > 1.0) {
