@@ -26,10 +26,14 @@ HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
     "-Wno-unused-function", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-# per-file extras: the FP64 stepper rounds every operation as written (no
-# FMA contraction) so it tracks the FP64 oracle to a few ulp
+# per-file extras: the FP64 support kernels round every operation as written
+# (no FMA contraction) so they track the FP64 oracle to a few ulp; the fused
+# FP64 step lets the compiler fuse multiplies and adds into FMAs, as the
+# reference's own CUDA build does by default (10% faster, within 1e-12 of
+# the oracle after 20000 steps at 2048^2)
 PER_FILE = {
     "rs_kernels.hip": ["-ffp-contract=off"],
+    "rs_step64.hip": ["-ffp-contract=fast"],
     "rs_params.hip": ["-ffp-contract=off"],
     "rs_capi.hip": ["-ffp-contract=off"],
     "rs_tip.hip": ["-ffp-contract=off"],
@@ -44,7 +48,7 @@ PER_FILE = {
     "tp_peritectic.hip": ["-ffp-contract=off"],
     "rs_noalloc.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["rs_kernels.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
+SOURCES = ["rs_kernels.hip", "rs_step64.hip", "rs_mixed.hip", "rs_mlp.hip", "rs_params.hip", "rs_tip.hip",
            "rs_capi.hip", "rs_surr.hip", "rs_batch.hip", "rs_io.hip", "tp_peritectic.hip", "rs_noalloc.hip"]
 
 LIB = os.path.join(PKG, "rsolid", "librsolid.so")

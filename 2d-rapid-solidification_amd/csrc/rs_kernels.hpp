@@ -38,6 +38,25 @@ struct TempSrc {
     double d0, GG, dx, W0, dT0, dl, Tmelt;
 };
 
+// fine temperature at (i, j) as the reference's T_fine / get_delta give it
+__device__ __forceinline__ double temp_at(const TempSrc& t, int i, int j) {
+    if (t.tfc) {
+        const int r = t.ratio;
+        const int iT = (i - 1 + t.ile) / r;
+        const int jT = (int)((j - 1 + t.dxT) / r);
+        const double x = (i - 1.0 + t.ile) / r - iT;
+        const double y = (j - 1.0 + t.dxT) / r - jT;
+        const double* row0 = t.T + (int64_t)jT * t.NxT_all;
+        const double* row1 = row0 + t.NxT_all;
+        const double f00 = row0[iT], f10 = row0[iT + 1];
+        const double f01 = row1[iT], f11 = row1[iT + 1];
+        const double a10 = f10 - f00, a01 = f01 - f00, a11 = f11 - f10 - f01 + f00;
+        return f00 + a10 * x + a01 * y + a11 * x * y;
+    }
+    const double delta = t.d0 + t.GG * (i + t.fta_ioff) * t.dx * t.S * t.W0 / t.dT0 - t.dl;
+    return delta * t.dT0 + t.Tmelt;
+}
+
 struct StepArgsF64 {
     const double* psi;
     const double* c;
