@@ -64,9 +64,10 @@ __device__ __forceinline__ Cell make_cell(const StepArgsF64& a, const double* ln
     return q;
 }
 
-// three wavefronts per SIMD: with every division and log as the reference
-// writes them, 537 us for a 4096^2 step on a developed front, against 598 us
-// at two (181 VGPRs) and 720 us at four (128 VGPRs, 50 registers spilled)
+// three wavefronts per SIMD (168 VGPRs, 12 spilled): 307 us for a 4096^2
+// step on a developed front, 371 us at two (191 VGPRs, none spilled); with
+// every division and log as the reference writes them and no contraction,
+// 537 us at three against 598 us at two and 720 us at four
 __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
     // the log table of the CALPHAD evaluation (ln_tab), staged in LDS
     __shared__ double lntab[2 * RS_LN_TAB_N];
@@ -95,10 +96,6 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
     Cell m = make_cell(a, lntab, a.psi[(int64_t)(jb - 1) * Nx + ic],
                        a.c[(int64_t)(jb - 1) * Nx + ic], ic, jb - 1);
     Cell z = make_cell(a, lntab, a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic], ic, jb);
-    double phL_m = shfl_up1(m.phi), phR_m = shfl_dn1(m.phi);
-    double SL_m = shfl_up1(m.S), SR_m = shfl_dn1(m.S);
-    double phL_z = shfl_up1(z.phi), phR_z = shfl_dn1(z.phi);
-    double SL_z = shfl_up1(z.S), SR_z = shfl_dn1(z.S);
 
     // prefetch of row jb+1
     double nx_psi = a.psi[(int64_t)(jb + 1) * Nx + ic];
@@ -113,6 +110,13 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
             nx_c = a.c[(int64_t)jn * Nx + ic];
         }
         const Cell p = make_cell(a, lntab, cur_psi, cur_c, ic, j + 1);
+        // the i+-1 neighbours of all three rows, shifted anew each row (eight
+        // more DPP moves; carried from row to row they held 16 more VGPRs,
+        // which spilled)
+        const double phL_m = shfl_up1(m.phi), phR_m = shfl_dn1(m.phi);
+        const double SL_m = shfl_up1(m.S), SR_m = shfl_dn1(m.S);
+        const double phL_z = shfl_up1(z.phi), phR_z = shfl_dn1(z.phi);
+        const double SL_z = shfl_up1(z.S), SR_z = shfl_dn1(z.S);
         const double phL_p = shfl_up1(p.phi), phR_p = shfl_dn1(p.phi);
         const double SL_p = shfl_up1(p.S), SR_p = shfl_dn1(p.S);
         const double muL = shfl_up1(z.mu), muR = shfl_dn1(z.mu);
@@ -191,9 +195,7 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
         }
         // slide the window
         m = z;
-        phL_m = phL_z; phR_m = phR_z; SL_m = SL_z; SR_m = SR_z;
         z = p;
-        phL_z = phL_p; phR_z = phR_p; SL_z = SL_p; SR_z = SR_p;
     }
 }
 
