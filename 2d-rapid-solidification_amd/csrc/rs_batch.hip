@@ -42,6 +42,13 @@ struct rs_batch {
     double last_ms = 0.0;
     int64_t last_steps = 0;
     std::vector<double> tune_log;
+    // surrogate batches: the network in every row (the unbranched kernel)
+    // while fewer than 1 row in 50 goes without it, decided from the runs'
+    // row counts after the first 64 batch steps and every 1024 after, as a
+    // single run decides (64-run ensemble: 35.6k -> 37.5k MLUPS)
+    int always = 0;
+    int64_t steps = 0, policy_at = 64;
+    unsigned long long seen_skip = 0, seen_rows = 0;
 };
 
 namespace {
@@ -180,8 +187,15 @@ int build_tables(rs_batch* b) {
     if (L.led_any != (led_any ? 1 : 0)) b->dirty = true;
     L.led_any = led_any ? 1 : 0;
     const rs_ctx* c0 = b->runs[0];
+    const int surr_was = L.surr;
     L.surr = c0->surrogate ? (c0->surr && surr_ready(c0) ? 1 : 0) : 0;
-    if (L.surr) L.surr = 1 + b->h_runs[0].a.s_fmt;
+    if (L.surr) {
+        // RS_SURR_ALWAYS (the run's override) or the batch's own decision
+        const int always = c0->surr_always >= 0 ? c0->surr_always : b->always;
+        const int fmt = b->h_runs[0].a.s_fmt;
+        L.surr = fmt == kActFp16Pk && always ? 8 : 1 + fmt;
+    }
+    if (L.surr != surr_was) b->dirty = true;
     const int res = batch_step_resident(L);
     L.step_blocks = std::max(1, std::min(L.items, res * b->cus * b->grid_mult));
     L.T_blocks = std::max(1, std::min((L.cells + 255) / 256, 8 * b->cus));
@@ -254,6 +268,25 @@ int run_segment(rs_batch* b, int64_t n) {
     return pull_state(b);
 }
 
+// the batch's network-in-every-row decision from the runs' row counts
+int surr_policy(rs_batch* b) {
+    unsigned long long skip_all = 0, rows_all = 0;
+    for (rs_ctx* c : b->runs) {
+        unsigned long long skip, rows;
+        const int rc = surr_row_counts(c, &skip, &rows);
+        if (rc) return bfail(b, rc, c->err.c_str());
+        skip_all += skip;
+        rows_all += rows;
+    }
+    if (rows_all < b->seen_rows || skip_all < b->seen_skip) b->seen_rows = b->seen_skip = 0;
+    const unsigned long long rows = rows_all - b->seen_rows, skip = skip_all - b->seen_skip;
+    if (rows < 4096) return RS_OK;
+    b->always = skip * 50 < rows ? 1 : 0;
+    b->seen_rows = rows_all;
+    b->seen_skip = skip_all;
+    return RS_OK;
+}
+
 int do_steps(rs_batch* b, int64_t nsteps) {
     for (rs_ctx* c : b->runs) BCHK(b, hipStreamSynchronize(c->stream));
     int64_t done = 0;
@@ -270,10 +303,18 @@ int do_steps(rs_batch* b, int64_t nsteps) {
                 BCHK(b, hipStreamSynchronize(c->stream));
             }
         }
-        const int64_t seg = std::min(nsteps - done, steps_to_pull_back(b));
+        const bool surr = b->runs[0]->surrogate && surr_ready(b->runs[0]);
+        int64_t seg = std::min(nsteps - done, steps_to_pull_back(b));
+        if (surr && b->steps < b->policy_at) seg = std::min(seg, b->policy_at - b->steps);
         int rc = run_segment(b, seg);
         if (rc) return rc;
         done += seg;
+        b->steps += seg;
+        if (surr && b->steps >= b->policy_at) {
+            rc = surr_policy(b);
+            if (rc) return rc;
+            b->policy_at = b->steps + 1024;
+        }
     }
     return RS_OK;
 }

@@ -47,6 +47,9 @@ uint64_t surr_identity(const rs_ctx* ctx);
 const uint64_t* surr_trust_host(const rs_ctx* ctx);
 void surr_step_args(rs_ctx* ctx, StepArgsMixed& a);
 int surr_update_policy(rs_ctx* ctx);
+// rows without the network (skip) and all rows the step kernel counted since
+// the counters were last cleared (waits for the context's stream)
+int surr_row_counts(rs_ctx* ctx, unsigned long long* skip, unsigned long long* rows);
 
 }  // namespace rs
 

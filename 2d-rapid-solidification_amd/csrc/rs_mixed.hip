@@ -916,7 +916,9 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
     case 6: e = batch_step_t<false, 3>(L, s); break;
     case 7: e = batch_step_t<true, 3>(L, s); break;
     case 8: e = batch_step_t<false, 4>(L, s); break;
-    default: e = batch_step_t<true, 4>(L, s); break;
+    case 9: e = batch_step_t<true, 4>(L, s); break;
+    case 16: e = batch_step_t<false, 8>(L, s); break;
+    default: e = batch_step_t<true, 8>(L, s); break;
     }
     if (e != hipSuccess) return e;
     if (L.at_any) {
@@ -956,7 +958,9 @@ int batch_step_resident(const BatchLaunch& L) {
     case 6: return resident_t<false, 3>(L.wpb, L.led_any != 0);
     case 7: return resident_t<true, 3>(L.wpb, L.led_any != 0);
     case 8: return resident_t<false, 4>(L.wpb, L.led_any != 0);
-    default: return resident_t<true, 4>(L.wpb, L.led_any != 0);
+    case 9: return resident_t<true, 4>(L.wpb, L.led_any != 0);
+    case 16: return resident_t<false, 8>(L.wpb, L.led_any != 0);
+    default: return resident_t<true, 8>(L.wpb, L.led_any != 0);
     }
 }
 

@@ -277,6 +277,21 @@ void surr_step_args(rs_ctx* ctx, StepArgsMixed& a) {
 // in every row (same results: such rows discard it), and go back to the
 // branch when the share rises again. Waits for the stream (it reads the
 // counts), so rs_step calls it once per 1024 surrogate steps.
+int surr_row_counts(rs_ctx* ctx, unsigned long long* skip, unsigned long long* rows) {
+    SurrState* s = ctx->surr;
+    *skip = *rows = 0;
+    if (!s || !s->ready) return RS_OK;
+    std::vector<unsigned long long> slots(kCountBytes / sizeof(unsigned long long));
+    HIPCHK(ctx, hipMemcpyAsync(slots.data(), s->count, kCountBytes, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned long long c[4];
+    sum_counts(slots.data(), c);
+    *skip = c[2];
+    *rows = c[3];
+    return RS_OK;
+}
+
 int surr_update_policy(rs_ctx* ctx) {
     SurrState* s = ctx->surr;
     if (!s || !s->ready) return RS_OK;
