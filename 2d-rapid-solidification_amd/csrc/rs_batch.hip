@@ -608,10 +608,11 @@ int rs_batch_info(rs_batch* b, double* out, int32_t n) {
     for (rs_ctx* c : b->runs) cells += (int64_t)c->p.Nx * c->p.Ny;
     int rc = build_tables(b);
     if (rc) return rc;
-    const double v[9] = {(double)b->runs.size(), (double)cells, (double)b->L.items,
-                         (double)b->L.step_blocks, (double)b->L.wpb, (double)b->graph_steps,
-                         b->last_ms, (double)b->last_steps, (double)b->grid_mult};
-    for (int q = 0; q < n && q < 9; ++q) out[q] = v[q];
+    const double v[10] = {(double)b->runs.size(), (double)cells, (double)b->L.items,
+                          (double)b->L.step_blocks, (double)b->L.wpb, (double)b->graph_steps,
+                          b->last_ms, (double)b->last_steps, (double)b->grid_mult,
+                          (double)b->L.surr};
+    for (int q = 0; q < n && q < 10; ++q) out[q] = v[q];
     return RS_OK;
 }
 

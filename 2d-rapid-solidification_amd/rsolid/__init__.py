@@ -487,8 +487,8 @@ class Batch:
         return {"ms_per_step": v[0], "grid_mult": int(v[1]), "band_h": grids}
 
     def info(self) -> dict:
-        v = (C.c_double * 9)()
-        self._chk(lib.rs_batch_info(self._h, v, 9), "rs_batch_info")
+        v = (C.c_double * 10)()
+        self._chk(lib.rs_batch_info(self._h, v, 10), "rs_batch_info")
         keys = ("runs", "cells", "items", "blocks", "waves_per_block", "graph_steps", "last_ms",
-                "last_steps", "grid_mult")
+                "last_steps", "grid_mult", "step_variant")
         return dict(zip(keys, list(v)))

@@ -476,7 +476,9 @@ int rs_batch_surrogate_train(rs_batch* b, const rs_surrogate_opts* o, double* re
 int rs_batch_tune(rs_batch* b, int32_t steps, double* out, int32_t n);
 /* out: [0] runs, [1] cells (sum of Nx*Ny), [2] work items, [3] persistent
  * blocks, [4] waves per block, [5] steps per captured graph, [6] ms of the
- * last rs_batch_step (HIP events), [7] its steps, [8] grid multiplier. */
+ * last rs_batch_step (HIP events), [7] its steps, [8] grid multiplier,
+ * [9] the step kernel's thermodynamics variant (0 exact table, 1-4 the
+ * surrogate formats, 8 packed FP16 with the network in every row). */
 int rs_batch_info(rs_batch* b, double* out, int32_t n);
 
 #ifdef __cplusplus

@@ -143,3 +143,46 @@ def test_batch_surrogate_matches_single_runs(rs):
     say("fallback per run", fb)
     _check(b, _singles(rs, params, seeds, 150, surrogate=weights))
     b.close()
+
+
+def test_batch_surrogate_every_row_kernel_changes_no_bit(rs, monkeypatch):
+    """After its first 64 steps a surrogate batch takes the kernel that
+    evaluates the network in every row while fewer than 1 row in 50 goes
+    without it (rs_batch_info out[9] = 8); the network then also runs in
+    rows where no cell uses it, so the fields equal bit for bit those of the
+    same batch held to the branched kernel by RS_SURR_ALWAYS=0 (variant 4)."""
+    params = [rs.ensemble_params(n, dT, v, RS_PREC_SURROGATE) for n, dT, v in SPECS[:3]]
+    seeds = [41, 42, 43]
+    weights = None
+    outs, variants, skips = [], [], []
+    for env in (None, "0"):
+        monkeypatch.delenv("RS_SURR_ALWAYS", raising=False)
+        if env is not None:
+            monkeypatch.setenv("RS_SURR_ALWAYS", env)
+        b = rs.Batch(params)
+        for run, s in zip(b.runs, seeds):
+            run.init_config(s)
+        if weights is None:
+            b.surrogate_train()
+            weights = b.runs[0].surrogate_get()
+        else:
+            for run in b.runs:
+                run.surrogate_set(*weights)
+        b.step(200)
+        variants.append(int(b.info()["step_variant"]))
+        st = [r.surrogate_stats() for r in b.runs]
+        skips.append(sum(x["rows_without_network"] for x in st) / sum(x["rows"] for x in st))
+        outs.append([_fields(r) for r in b.runs])
+        b.close()
+    monkeypatch.delenv("RS_SURR_ALWAYS", raising=False)
+    say("variants", variants, "rows without the network", skips)
+    assert variants[1] == 4
+    # the decision reads the counts of steps 0-63; well inside the 1-in-50
+    # rule over the whole run it must have switched
+    assert variants[0] in (4, 8)
+    if skips[0] < 0.005:
+        assert variants[0] == 8
+    for q, (a, r) in enumerate(zip(outs[0], outs[1])):
+        for x, y in zip(a, r):
+            assert np.isfinite(x).all()
+            assert np.array_equal(x, y), q
