@@ -513,6 +513,10 @@ int rs_batch_surrogate_train(rs_batch* b, const rs_surrogate_opts* o, double* re
         rc = rs_surrogate_set(b->runs[r], P.data(), RS_SURR_NPARAM, meta, RS_SURR_NMETA);
     if (rc) return bfail(b, rc, "installing the shared surrogate");
     b->dirty = true;
+    // a new network: the every-row choice starts over from its own counts
+    b->always = 0;
+    b->policy_at = b->steps + 64;
+    b->seen_skip = b->seen_rows = 0;
     return RS_OK;
 }
 
