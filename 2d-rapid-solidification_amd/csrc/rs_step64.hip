@@ -64,7 +64,7 @@ __device__ __forceinline__ Cell make_cell(const StepArgsF64& a, const double* ln
     return q;
 }
 
-// three wavefronts per SIMD (168 VGPRs, 12 spilled): 307 us for a 4096^2
+// three wavefronts per SIMD (168 VGPRs, 16 spilled): 303 us for a 4096^2
 // step on a developed front, 371 us at two (191 VGPRs, none spilled); with
 // every division and log as the reference writes them and no contraction,
 // 537 us at three against 598 us at two and 720 us at four
@@ -93,23 +93,25 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
     const double idG = 1.0 / a.dG_den, idx2 = 1.0 / a.dx2;
 
     // rows jb-1 and jb
-    Cell m = make_cell(a, lntab, a.psi[(int64_t)(jb - 1) * Nx + ic],
-                       a.c[(int64_t)(jb - 1) * Nx + ic], ic, jb - 1);
-    Cell z = make_cell(a, lntab, a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic], ic, jb);
+    Cell s0 = make_cell(a, lntab, a.psi[(int64_t)(jb - 1) * Nx + ic],
+                        a.c[(int64_t)(jb - 1) * Nx + ic], ic, jb - 1);
+    Cell s1 = make_cell(a, lntab, a.psi[(int64_t)jb * Nx + ic], a.c[(int64_t)jb * Nx + ic], ic, jb);
+    Cell s2;
 
     // prefetch of row jb+1
     double nx_psi = a.psi[(int64_t)(jb + 1) * Nx + ic];
     double nx_c = a.c[(int64_t)(jb + 1) * Nx + ic];
     double lh_acc = 0.0;
 
-    for (int j = jb; j <= je; ++j) {
+    // output row j from rows m = j-1, z = j and p = j+1 (filled here)
+    auto row = [&](int j, const Cell& m, const Cell& z, Cell& p) {
         const double cur_psi = nx_psi, cur_c = nx_c;
         {  // unconditional, the last row reloading row je+1 (no wait on the stores)
             const int jn = min(j + 2, je + 1);
             nx_psi = a.psi[(int64_t)jn * Nx + ic];
             nx_c = a.c[(int64_t)jn * Nx + ic];
         }
-        const Cell p = make_cell(a, lntab, cur_psi, cur_c, ic, j + 1);
+        p = make_cell(a, lntab, cur_psi, cur_c, ic, j + 1);
         // the i+-1 neighbours of all three rows, shifted anew each row (eight
         // more DPP moves; carried from row to row they held 16 more VGPRs,
         // which spilled)
@@ -193,9 +195,15 @@ __global__ __launch_bounds__(256, 3) void step_fp64_kernel(StepArgsF64 a) {
                 }
             }
         }
-        // slide the window
-        m = z;
-        z = p;
+    };
+    // three rows per trip with the window's roles rotated: no register
+    // copies of the window between rows
+    for (int j = jb; j <= je; j += 3) {
+        row(j, s0, s1, s2);
+        if (j + 1 > je) break;
+        row(j + 1, s1, s2, s0);
+        if (j + 2 > je) break;
+        row(j + 2, s2, s0, s1);
     }
 }
 
