@@ -108,8 +108,8 @@ def test_surrogate_4096_long_horizon_vs_fp64_path(rs):
 
 
 def test_precision_4096_vs_fp64_path(rs):
-    """configs[2] (FP32 psi/phi and T, FP64 c) and configs[3] (the surrogate
-    path) at 4096^2 beside the plain FP64 path over 6 000 steps, compared at
+    """The precision variant (FP32 psi/phi and T, FP64 c) and the surrogate
+    path at 4096^2 beside the plain FP64 path over 6 000 steps, compared at
     3 000 and 6 000 by the precision report (rsolid.report; its 50 000-step
     run is the profiles' precision_4096.md): per-field errors, tip and front
     velocity over each interval, and each path's solute ledger. Bounds: FP32
